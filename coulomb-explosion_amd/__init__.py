@@ -1,0 +1,390 @@
+"""Host-side mirror of the reference's batched f32 engine, over the engine's C-ABI.
+
+The classes keep the reference's names and argument meaning:
+
+* ``Atom`` / ``atom_from_amu`` / ``Molecule``  -- include/coulomb/molecule.hpp:14-35
+* ``RunConfig``                               -- include/coulomb/driver.hpp:28-50 (+ the
+  engine's stop rule, t_max, momentum_tol, fixed h0, precision)
+* ``UniformSphereSampler``                    -- include/coulomb/sampler.hpp:46-72 (host
+  mt19937_64 path of the reference, same stream as the reference for a seed)
+* ``BatchedRK45``                             -- include/coulomb/batched_integrator.hpp:62-191
+  (``set_lane_geometry``, ``run`` -> ``Result{momenta, steps, converged, batch_steps}``)
+
+Errors follow the reference: ``ValueError`` where it throws std::invalid_argument,
+``RuntimeError`` where it throws std::runtime_error.  The library
+(libcoulomb_explosion.so, built in this directory) is required: importing this
+module fails loudly when it is missing, and nothing here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcoulomb_explosion.so")
+HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "coulomb_explosion.h")
+
+MAX_ATOMS = 10
+AMU_TO_ELECTRON_MASS = 1822.888486209  # CODATA 2018 m_u/m_e (include/coulomb/units.hpp:11)
+
+STOP_PE_FRACTION = 0
+STOP_MOMENTUM_OR_TMAX = 1
+H0_HAIRER = 0
+H0_FIXED = 1
+PRECISION_F32 = 0
+PRECISION_F64 = 1
+
+SIM_OK_PE, SIM_OK_MOMENTUM, SIM_OK_TMAX, SIM_NOT_CONVERGED = 0, 1, 2, 3
+SIM_REJECT_LIMIT, SIM_NONFINITE, SIM_SAMPLER_FAIL = 4, 5, 6
+
+CE_OK, CE_ERR_INVALID_ARGUMENT, CE_ERR_RUNTIME, CE_ERR_HIP, CE_ERR_NO_DEVICE = 0, -1, -2, -3, -4
+
+
+# ------------------------------------------------------------------ C types ----
+class _MoleculeDesc(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("pad_", C.c_int32), ("masses", C.POINTER(C.c_double)),
+                ("charges", C.POINTER(C.c_double)), ("coulomb_k", C.c_double)]
+
+
+class _Tolerances(C.Structure):
+    _fields_ = [("rtol", C.c_double), ("atol", C.c_double)]
+
+
+class _RunConfig(C.Structure):
+    _fields_ = [("pe_stop_fraction", C.c_double), ("max_dt", C.c_double), ("t_max", C.c_double),
+                ("momentum_tol", C.c_double), ("h0", C.c_double), ("max_steps", C.c_uint64),
+                ("stop_rule", C.c_int32), ("redistribute_energy", C.c_int32),
+                ("h0_mode", C.c_int32), ("max_rejections", C.c_int32), ("precision", C.c_int32),
+                ("pad_", C.c_int32)]
+
+
+class _Results(C.Structure):
+    _fields_ = [("momenta", C.POINTER(C.c_double)), ("momenta_f32", C.POINTER(C.c_float)),
+                ("steps", C.POINTER(C.c_uint32)), ("attempts", C.POINTER(C.c_uint32)),
+                ("status", C.POINTER(C.c_uint8)), ("t_final", C.POINTER(C.c_double)),
+                ("energy_err", C.POINTER(C.c_double)), ("trace_h", C.POINTER(C.c_double)),
+                ("trace_cap", C.c_uint32), ("pad_", C.c_uint32)]
+
+
+class _RunStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("sampler_ms", C.c_double), ("sims", C.c_uint64),
+                ("attempted_steps", C.c_uint64), ("accepted_steps", C.c_uint64),
+                ("refills", C.c_uint64), ("grid_blocks", C.c_int32),
+                ("block_threads", C.c_int32), ("n_cus", C.c_int32), ("pad_", C.c_int32)]
+
+
+class _LaunchConfig(C.Structure):
+    _fields_ = [("block_threads", C.c_int32), ("blocks_per_cu", C.c_int32),
+                ("refill_threshold", C.c_int32), ("chunk", C.c_int32)]
+
+
+def _load_library(path: str = LIB_PATH) -> C.CDLL:
+    if not os.path.exists(path):
+        raise ImportError(
+            f"{path} is not built; run `make -C coulomb-explosion_amd` (or "
+            "__graft_entry__.build()).  There is no CPU fallback.")
+    lib = C.CDLL(path)
+    P, S = C.c_void_p, C.c_size_t
+    sigs = {
+        "ce_last_error": ([], C.c_char_p),
+        "ce_device_count": ([P], C.c_int),
+        "ce_run_config_default": ([P], None),
+        "ce_engine_create": ([P, P, S, P], C.c_int),
+        "ce_engine_destroy": ([P], C.c_int),
+        "ce_engine_set_geometries": ([P, P, S], C.c_int),
+        "ce_engine_sample": ([P, C.c_uint64, C.c_uint64, S, C.c_double, C.c_double, C.c_uint32],
+                             C.c_int),
+        "ce_engine_get_geometries": ([P, P, S], C.c_int),
+        "ce_engine_run": ([P, P, P], C.c_int),
+        "ce_engine_download": ([P, P], C.c_int),
+        "ce_engine_stats": ([P, P], C.c_int),
+        "ce_engine_set_launch": ([P, P], C.c_int),
+        "ce_engine_get_launch": ([P, P], C.c_int),
+        "ce_engine_enable_trace": ([P, C.c_uint32], C.c_int),
+        "ce_host_sampler_create": ([C.c_uint64, C.c_double, C.c_double, C.c_uint64, P], C.c_int),
+        "ce_host_sampler_reseed": ([P, C.c_uint64], C.c_int),
+        "ce_host_sampler_sample": ([P, C.c_int32, S, P], C.c_int),
+        "ce_host_sampler_destroy": ([P], C.c_int),
+        "ce_batched_accelerations_f32": ([S, S, P, P, P, P, P, P, P, P], C.c_int),
+        "ce_batched_potential_energy_f32": ([S, S, P, P, P, P, P], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    return lib
+
+
+_lib = _load_library()
+
+
+def library() -> C.CDLL:
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc == CE_OK:
+        return
+    msg = (_lib.ce_last_error() or b"").decode(errors="replace")
+    if rc == CE_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    raise RuntimeError(f"coulomb engine error {rc}: {msg}")
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _lib.ce_device_count(C.byref(n))
+    return n.value
+
+
+# --------------------------------------------------------- reference mirrors ----
+@dataclass
+class Atom:
+    symbol: str
+    mass: float    # electron masses
+    charge: float
+
+
+def atom_from_amu(symbol: str, mass_amu: float, charge: float) -> Atom:
+    """include/coulomb/molecule.hpp:23-25"""
+    return Atom(symbol, mass_amu * AMU_TO_ELECTRON_MASS, charge)
+
+
+@dataclass
+class Molecule:
+    atoms: list = field(default_factory=list)
+
+    def size(self) -> int:
+        return len(self.atoms)
+
+    @property
+    def masses(self) -> np.ndarray:
+        return np.array([a.mass for a in self.atoms], dtype=np.float64)
+
+    @property
+    def charges(self) -> np.ndarray:
+        return np.array([a.charge for a in self.atoms], dtype=np.float64)
+
+
+@dataclass
+class RunConfig:
+    """include/coulomb/driver.hpp:28-50 defaults, plus the engine's fields."""
+    pe_stop_fraction: float = 1e-9
+    redistribute_energy: bool = True
+    max_dt: float = 1e12
+    max_steps: int = 1_000_000
+    stop_rule: int = STOP_PE_FRACTION
+    t_max: float = 1e4
+    momentum_tol: float = 1e-7
+    h0_mode: int = H0_HAIRER
+    h0: float = 1e-3
+    max_rejections: int = 100
+    precision: int = PRECISION_F32
+
+    def _c(self) -> _RunConfig:
+        return _RunConfig(self.pe_stop_fraction, self.max_dt, self.t_max, self.momentum_tol,
+                          self.h0, self.max_steps, self.stop_rule, int(self.redistribute_energy),
+                          self.h0_mode, self.max_rejections, self.precision, 0)
+
+
+@dataclass
+class Result:
+    """BatchedRK45::Result (batched_integrator.hpp:64-69) plus per-sim extras."""
+    momenta: np.ndarray      # float64 [(lane*n + i)*3 + c], flat like the reference
+    steps: np.ndarray        # accepted steps per sim
+    converged: np.ndarray    # uint8: 1 unless not-converged / failed
+    batch_steps: int         # max accepted steps over the sims (no lockstep here)
+    status: np.ndarray
+    attempts: np.ndarray
+    t_final: np.ndarray
+    energy_err: np.ndarray
+    momenta_f32: np.ndarray
+    trace_h: np.ndarray | None = None
+
+
+class UniformSphereSampler:
+    """UniformSphereSampler(Options, seed) (include/coulomb/sampler.hpp:46-72).
+
+    Same stream as the reference for a seed (std::mt19937_64 +
+    uniform_real_distribution, in the engine library's host code); stateful,
+    so successive ``sample`` calls continue the stream.
+    """
+
+    def __init__(self, radius: float = 4.0, min_separation: float = 0.25,
+                 max_attempts_per_atom: int = 10000, seed: int = 0):
+        h = C.c_void_p()
+        _check(_lib.ce_host_sampler_create(seed, radius, min_separation, max_attempts_per_atom,
+                                           C.byref(h)))
+        self._h = h
+        self.radius = float(radius)
+        self.min_separation = float(min_separation)
+        self.max_attempts_per_atom = int(max_attempts_per_atom)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.ce_host_sampler_destroy(h)
+            self._h = None
+
+    def name(self) -> str:
+        return "uniform-sphere"
+
+    def reseed(self, seed: int) -> None:
+        _check(_lib.ce_host_sampler_reseed(self._h, seed))
+
+    def sample(self, mol_or_n, n_sims: int = 1) -> np.ndarray:
+        """n_sims consecutive draws -> positions [n_sims, n, 3] (atoms at rest)."""
+        n = mol_or_n.size() if isinstance(mol_or_n, Molecule) else int(mol_or_n)
+        out = np.zeros((n_sims, n, 3), dtype=np.float64)
+        _check(_lib.ce_host_sampler_sample(self._h, n, n_sims, out.ctypes.data))
+        return out
+
+
+class BatchedRK45:
+    """Drop-in for BatchedRK45<float> (batched_integrator.hpp:62-191).
+
+    ``lanes`` is the batch capacity (the reference's SIMD width; any size here).
+    """
+
+    def __init__(self, mol: Molecule, coulomb_k: float, lanes: int, rtol: float, atol: float):
+        if mol.size() < 1 or mol.size() > MAX_ATOMS:
+            raise ValueError(f"BatchedRK45: molecule must have 1..{MAX_ATOMS} atoms")
+        if lanes < 1:
+            raise ValueError("BatchedRK45: lanes must be positive")
+        self.n = mol.size()
+        self.k = int(lanes)
+        self._m = np.ascontiguousarray(mol.masses)
+        self._q = np.ascontiguousarray(mol.charges)
+        desc = _MoleculeDesc(self.n, 0, self._m.ctypes.data_as(C.POINTER(C.c_double)),
+                             self._q.ctypes.data_as(C.POINTER(C.c_double)), float(coulomb_k))
+        tol = _Tolerances(float(rtol), float(atol))
+        h = C.c_void_p()
+        _check(_lib.ce_engine_create(C.byref(desc), C.byref(tol), self.k, C.byref(h)))
+        self._h = h
+        self._geom = np.zeros((self.k, self.n, 3), dtype=np.float64)
+        self._loaded = 0
+        self._dirty = False
+        self._trace_cap = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.ce_engine_destroy(h)
+            self._h = None
+
+    def lanes(self) -> int:
+        return self.k
+
+    def atoms(self) -> int:
+        return self.n
+
+    # -- geometry ---------------------------------------------------------------
+    def set_lane_geometry(self, lane: int, positions) -> None:
+        """batched_integrator.hpp:112-121: positions (n x 3), velocities at rest."""
+        if not 0 <= lane < self.k:
+            raise ValueError("lane out of range")
+        self._geom[lane] = np.asarray(positions, dtype=np.float64).reshape(self.n, 3)
+        self._loaded = self.k
+        self._dirty = True
+
+    def set_geometries(self, pos) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.n, 3)
+        if pos.shape[0] > self.k:
+            raise ValueError("more geometries than lanes")
+        self._geom[: pos.shape[0]] = pos
+        self._loaded = pos.shape[0]
+        _check(_lib.ce_engine_set_geometries(self._h, self._geom.ctypes.data, self._loaded))
+        self._dirty = False
+
+    def sample(self, seed: int, n_sims: int, radius: float, min_separation: float,
+               first_index: int = 0, max_attempts: int = 10000) -> None:
+        """Device-side counter-based uniform-sphere sampler."""
+        if n_sims > self.k:
+            raise ValueError("n_sims exceeds lanes")
+        _check(_lib.ce_engine_sample(self._h, seed, first_index, n_sims, radius, min_separation,
+                                     max_attempts))
+        self._loaded = int(n_sims)
+        self._dirty = False
+
+    def geometries(self) -> np.ndarray:
+        out = np.zeros((self._loaded, self.n, 3), dtype=np.float64)
+        if self._loaded:
+            _check(_lib.ce_engine_get_geometries(self._h, out.ctypes.data, self._loaded))
+        return out
+
+    # -- configuration ------------------------------------------------------------
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0) -> None:
+        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk)
+        _check(_lib.ce_engine_set_launch(self._h, C.byref(lc)))
+
+    def launch(self) -> dict:
+        lc = _LaunchConfig()
+        _check(_lib.ce_engine_get_launch(self._h, C.byref(lc)))
+        return {f: getattr(lc, f) for f, _ in _LaunchConfig._fields_}
+
+    def enable_trace(self, cap: int) -> None:
+        _check(_lib.ce_engine_enable_trace(self._h, cap))
+        self._trace_cap = int(cap)
+
+    # -- run ----------------------------------------------------------------------
+    def run(self, cfg: RunConfig | None = None, download: bool = True) -> Result | None:
+        """batched_integrator.hpp:123-191."""
+        cfg = cfg or RunConfig()
+        if self._dirty:
+            _check(_lib.ce_engine_set_geometries(self._h, self._geom.ctypes.data, self._loaded))
+            self._dirty = False
+        c = cfg._c()
+        if not download:
+            _check(_lib.ce_engine_run(self._h, C.byref(c), None))
+            return None
+        _check(_lib.ce_engine_run(self._h, C.byref(c), None))
+        return self.download()
+
+    def download(self) -> Result:
+        ns, n = self._loaded, self.n
+        mom = np.zeros((ns, n, 3))
+        mom32 = np.zeros((ns, n, 3), dtype=np.float32)
+        steps = np.zeros(ns, np.uint32)
+        att = np.zeros(ns, np.uint32)
+        st = np.zeros(ns, np.uint8)
+        tf = np.zeros(ns)
+        ee = np.zeros(ns)
+        tr = np.zeros((ns, self._trace_cap)) if self._trace_cap else None
+
+        def ptr(a, ct):
+            return a.ctypes.data_as(C.POINTER(ct))
+
+        res = _Results(ptr(mom, C.c_double), ptr(mom32, C.c_float), ptr(steps, C.c_uint32),
+                       ptr(att, C.c_uint32), ptr(st, C.c_uint8), ptr(tf, C.c_double),
+                       ptr(ee, C.c_double), ptr(tr, C.c_double) if tr is not None else None,
+                       self._trace_cap, 0)
+        _check(_lib.ce_engine_download(self._h, C.byref(res)))
+        conv = (st <= SIM_OK_TMAX).astype(np.uint8)
+        return Result(mom.reshape(-1), steps, conv, int(steps.max()) if ns else 0, st, att, tf,
+                      ee, mom32, tr)
+
+    def stats(self) -> dict:
+        s = _RunStats()
+        _check(_lib.ce_engine_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _RunStats._fields_ if f != "pad_"}
+
+
+def batched_accelerations(n: int, k: int, px, py, pz, pair_const, inv_mass):
+    """batched::accelerations<float> (batched_force.hpp:31-84), same SoA layout."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (px, py, pz, pair_const, inv_mass)]
+    out = [np.zeros(n * k, dtype=np.float32) for _ in range(3)]
+    _check(_lib.ce_batched_accelerations_f32(n, k, *[a.ctypes.data for a in arrs],
+                                             *[o.ctypes.data for o in out]))
+    return out
+
+
+def batched_potential_energy(n: int, k: int, px, py, pz, pair_const):
+    """batched::potential_energy<float> (batched_force.hpp:89-110)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float32) for a in (px, py, pz, pair_const)]
+    pe = np.zeros(k, dtype=np.float32)
+    _check(_lib.ce_batched_potential_energy_f32(n, k, *[a.ctypes.data for a in arrs],
+                                                pe.ctypes.data))
+    return pe

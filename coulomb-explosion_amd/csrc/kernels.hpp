@@ -1,0 +1,668 @@
+// Device code of the engine: per-lane Coulomb force, the persistent fused
+// DP5(4) integrator with device work-queue refill, and the counter-based
+// uniform-sphere sampler.  gfx950, wave64.  Included by engine.hip only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <type_traits>
+
+#include "dp5_table.hpp"
+
+namespace ce {
+
+// ---------------------------------------------------------------- helpers ----
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float sqrtT(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ double sqrtT(double a) { return __builtin_sqrt(a); }
+__device__ __forceinline__ float powT(float a, float b) { return powf(a, b); }
+__device__ __forceinline__ double powT(double a, double b) { return pow(a, b); }
+// 1/sqrt(d2): the f32 path uses the hardware reciprocal square root
+// (v_rsq_f32, 1 ulp); the f64 oracle path the correctly rounded 1/sqrt.
+__device__ __forceinline__ float rsqrtT(float a) { return __builtin_amdgcn_rsqf(a); }
+__device__ __forceinline__ double rsqrtT(double a) { return 1.0 / __builtin_sqrt(a); }
+// 1/x inside the error norm: v_rcp_f32 (1 ulp) / exact in f64.
+__device__ __forceinline__ float rcpT(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ __forceinline__ double rcpT(double a) { return 1.0 / a; }
+// std::min / std::max semantics ((b < a) ? b : a), NaN in b keeps a.
+template <class T>
+__device__ __forceinline__ T minT(T a, T b) { return (b < a) ? b : a; }
+template <class T>
+__device__ __forceinline__ T maxT(T a, T b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ uint64_t bcast64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+// -------------------------------------------------------------- molecule ----
+
+constexpr int pairs_of(int n) { return n * (n - 1) / 2 > 0 ? n * (n - 1) / 2 : 1; }
+
+// Per-molecule constants, passed by value (kernel arguments -> scalar registers).
+template <class T, int N>
+struct MolConst {
+  T pc[pairs_of(N)];       // k q_i q_j narrowed from double, pair order (0,1),(0,2)..
+  T im[N];                 // 1/m_i narrowed from double
+  double pcd[pairs_of(N)]; // k q_i q_j in double (energy diagnostic)
+  double md[N];            // masses in double (finalize)
+};
+
+template <int N>
+constexpr int pair_index(int i, int j) {  // i < j
+  return i * N - i * (i + 1) / 2 + (j - i - 1);
+}
+
+// Accelerations a(x) of one lane (x, a: [3N] atom-major), optionally the PE.
+// Pair term: r = x_i - x_j, d2 = r.r (fma), inv = 1/|r|, s = k q_i q_j inv^3,
+// f = r s accumulated +f on i, -f on j (Newton 3), then a_i *= 1/m_i.
+template <class T, int N, bool WITH_PE>
+__device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T& pe,
+                                           const MolConst<T, N>& mc) {
+#pragma unroll
+  for (int q = 0; q < 3 * N; ++q) a[q] = T(0);
+  T u = T(0);
+  static_for<0, N>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    static_for<i + 1, N>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      constexpr int p = pair_index<N>(i, j);
+      const T rx = x[3 * i] - x[3 * j];
+      const T ry = x[3 * i + 1] - x[3 * j + 1];
+      const T rz = x[3 * i + 2] - x[3 * j + 2];
+      const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
+      const T inv = rsqrtT(d2);
+      const T inv3 = (inv * inv) * inv;
+      const T s = mc.pc[p] * inv3;
+      if constexpr (WITH_PE) u = fmaT(mc.pc[p], inv, u);
+      const T fx = rx * s, fy = ry * s, fz = rz * s;
+      a[3 * i] += fx;
+      a[3 * i + 1] += fy;
+      a[3 * i + 2] += fz;
+      a[3 * j] -= fx;
+      a[3 * j + 1] -= fy;
+      a[3 * j + 2] -= fz;
+    });
+  });
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    a[3 * i] *= mc.im[i];
+    a[3 * i + 1] *= mc.im[i];
+    a[3 * i + 2] *= mc.im[i];
+  }
+  if constexpr (WITH_PE) pe = u;
+}
+
+template <class T, int N>
+__device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const MolConst<T, N>& mc) {
+  T u = T(0);
+  static_for<0, N>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    static_for<i + 1, N>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      const T rx = x[3 * i] - x[3 * j];
+      const T ry = x[3 * i + 1] - x[3 * j + 1];
+      const T rz = x[3 * i + 2] - x[3 * j + 2];
+      const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
+      u = fmaT(mc.pc[pair_index<N>(i, j)], rsqrtT(d2), u);
+    });
+  });
+  return u;
+}
+
+// Sum_{l < L} coef(l) * k[l][q] with zero coefficients skipped at compile time;
+// the first non-zero term is a product, the rest are FMAs in ascending l.
+template <class T, int L, class CoefFn, int M>
+__device__ __forceinline__ T stage_sum(const T (&k)[7][M], int q, CoefFn coef) {
+  T acc = T(0);
+  bool first = true;
+  static_for<0, L>([&](auto Lc) {
+    constexpr int l = decltype(Lc)::value;
+    constexpr double c = CoefFn::template value<l>();
+    if constexpr (c != 0.0) {
+      if (first)
+        acc = static_cast<T>(c) * k[l][q];
+      else
+        acc = fmaT(static_cast<T>(c), k[l][q], acc);
+      first = false;
+    }
+  });
+  (void)coef;
+  return acc;
+}
+
+template <int S>
+struct CoefAA {
+  template <int L>
+  static constexpr double value() { return kDP5.aa[S][L]; }
+};
+struct CoefB {
+  template <int L>
+  static constexpr double value() { return kDP5.b[L]; }
+};
+struct CoefE {
+  template <int L>
+  static constexpr double value() { return kDP5.e[L]; }
+};
+struct CoefEA {
+  template <int L>
+  static constexpr double value() { return kDP5.ea[L]; }
+};
+
+// Hairer / scipy initial step (src/integrators.cpp:197-250) for one lane, in
+// the scalar reference's accumulation order; k0 = a(x0).
+template <class T, int N>
+__device__ T lane_hairer(const T (&x)[3 * N], const T (&v)[3 * N], const T (&a0)[3 * N], T rtol,
+                         T atol, T max_dt, const MolConst<T, N>& mc) {
+  T d0s = T(0), d1s = T(0);
+#pragma unroll
+  for (int q = 0; q < 3 * N; ++q) {
+    const T sx = atol + rtol * absT(x[q]);
+    const T sv = atol + rtol * absT(v[q]);
+    T r = x[q] / sx;
+    d0s += r * r;
+    r = v[q] / sv;
+    d0s += r * r;
+    r = v[q] / sx;
+    d1s += r * r;
+    r = a0[q] / sv;
+    d1s += r * r;
+  }
+  const T den = static_cast<T>(6 * N);
+  const T d0 = sqrtT(d0s / den);
+  const T d1 = sqrtT(d1s / den);
+  const T h0 = (d0 < T(1e-5) || d1 < T(1e-5)) ? T(1e-6) : T(0.01) * d0 / d1;
+  T xs[3 * N], a1[3 * N];
+#pragma unroll
+  for (int q = 0; q < 3 * N; ++q) xs[q] = x[q] + v[q] * h0;
+  T unused;
+  lane_force<T, N, false>(xs, a1, unused, mc);
+  T d2s = T(0);
+#pragma unroll
+  for (int q = 0; q < 3 * N; ++q) {
+    const T sx = atol + rtol * absT(x[q]);
+    const T sv = atol + rtol * absT(v[q]);
+    T r = (a0[q] * h0) / sx;
+    d2s += r * r;
+    r = (a1[q] - a0[q]) / sv;
+    d2s += r * r;
+  }
+  const T d2 = sqrtT(d2s / den) / h0;
+  const T h1 = (d1 <= T(1e-15) && d2 <= T(1e-15)) ? maxT(T(1e-6), h0 * T(1e-3))
+                                                   : powT(T(0.01) / maxT(d1, d2), T(0.2));
+  return minT(minT(T(100) * h0, h1), max_dt);
+}
+
+// ------------------------------------------------------------- integrator ----
+
+enum : int {
+  kStopPE = 0,
+  kStopMomentumOrTmax = 1,
+  kH0Hairer = 0,
+  kH0Fixed = 1,
+};
+enum : int {
+  kSimOkPE = 0,
+  kSimOkMomentum = 1,
+  kSimOkTmax = 2,
+  kSimNotConverged = 3,
+  kSimRejectLimit = 4,
+  kSimNonfinite = 5,
+  kSimSamplerFail = 6,
+  kRunning = 255,
+};
+
+struct RunArgs {
+  const double* geom;         // [sim][N][3]
+  const uint8_t* geom_status; // nullptr, or non-zero = sampler failure
+  uint64_t n_sims;
+  unsigned long long* ctr;    // [0] queue head [1] attempts [2] accepted [3] refills
+  double* mom;                // [sim][N][3]
+  float* mom32;               // [sim][N][3]
+  uint32_t* steps;
+  uint32_t* attempts;
+  uint8_t* status;
+  double* t_final;
+  double* eerr;
+  double* trace_h;            // [sim][trace_cap] or nullptr
+  uint32_t trace_cap;
+  int32_t stop_rule;
+  double rtol, atol, max_dt, t_max, pe_stop_fraction, momentum_tol, h0;
+  uint64_t max_steps;
+  int32_t redistribute;
+  int32_t h0_mode;
+  int32_t max_rej;
+  int32_t refill_threshold;   // 1..64
+  int32_t chunk;              // >= 64
+};
+
+// Persistent fused DP5(4): each lane integrates one simulation at a time with
+// its own step size; a wave pulls new simulation indices from a device work
+// queue (one atomic per `chunk` sims) whenever at least `refill_threshold` of
+// its lanes are idle, so converged lanes do not wait for stragglers.  Every
+// loop iteration is exactly one attempted step for every running lane, so the
+// wave stays convergent except for the short init / finalize branches.
+template <class T, int N>
+__global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
+  constexpr int N3 = 3 * N;
+  const T rtol = static_cast<T>(p.rtol), atol = static_cast<T>(p.atol);
+  const T max_dt = static_cast<T>(p.max_dt);
+  const T inv6n = T(1) / static_cast<T>(6 * N);
+  const T tol2 = static_cast<T>(p.momentum_tol * p.momentum_tol);
+  const int lane = __lane_id();
+
+  T x[N3], v[N3], k[7][N3];
+  T h = T(0);
+  double t = 0.0, e0 = 0.0, pe_stop = 0.0;
+  uint32_t steps = 0, att = 0;
+  int nrej = 0;
+  bool rej_once = false;
+  uint64_t sim = 0;
+  bool active = false;
+  int status = kRunning;
+  uint32_t tot_att = 0, tot_acc = 0, refills = 0;
+  uint64_t cbase = 0, cend = 0;
+  bool exhausted = false;
+
+  for (;;) {
+    // ---- refill idle lanes from the work queue (wave-uniform control) ----
+    const uint64_t idle = __ballot(!active);
+    const int n_idle = __popcll(idle);
+    if (!exhausted && n_idle >= p.refill_threshold) {
+      ++refills;
+      const int rank = static_cast<int>(lanes_below(idle));
+      int assigned = 0;
+      uint64_t mine = ~0ull;
+      while (assigned < n_idle) {
+        if (cbase >= cend) {
+          uint64_t b = 0;
+          if (lane == 0) b = atomicAdd(p.ctr, static_cast<unsigned long long>(p.chunk));
+          b = bcast64(b);
+          if (b >= p.n_sims) {
+            exhausted = true;
+            break;
+          }
+          cbase = b;
+          cend = (b + p.chunk < p.n_sims) ? b + p.chunk : p.n_sims;
+        }
+        const uint64_t left = cend - cbase;
+        const int take = (static_cast<uint64_t>(n_idle - assigned) < left)
+                             ? (n_idle - assigned)
+                             : static_cast<int>(left);
+        if (!active && rank >= assigned && rank < assigned + take)
+          mine = cbase + static_cast<uint64_t>(rank - assigned);
+        cbase += take;
+        assigned += take;
+      }
+      if (!active && mine != ~0ull) {
+        sim = mine;
+        active = true;
+        steps = 0;
+        att = 0;
+        nrej = 0;
+        rej_once = false;
+        t = 0.0;
+        status = kRunning;
+        const double* g = p.geom + sim * N3;
+#pragma unroll
+        for (int q = 0; q < N3; ++q) {
+          x[q] = static_cast<T>(g[q]);
+          v[q] = T(0);
+        }
+        if (p.geom_status != nullptr && p.geom_status[sim] != 0) {
+          status = kSimSamplerFail;
+          e0 = 0.0;
+        } else {
+          T pe0;
+          lane_force<T, N, true>(x, k[0], pe0, mc);
+          e0 = static_cast<double>(pe0);
+          pe_stop = p.pe_stop_fraction * e0;
+          if (p.stop_rule == kStopPE && e0 <= pe_stop) {
+            status = kSimOkPE;
+          } else if (p.max_steps == 0) {
+            status = kSimNotConverged;
+          } else {
+            h = (p.h0_mode == kH0Fixed) ? static_cast<T>(p.h0)
+                                        : lane_hairer<T, N>(x, v, k[0], rtol, atol, max_dt, mc);
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) break;
+
+    // ---- one attempted DP5(4) step on every running lane ----
+    if (active && status == kRunning) {
+      T cap = max_dt;
+      bool by_time = false;
+      if (p.stop_rule == kStopMomentumOrTmax) {
+        const double rem = p.t_max - t;
+        if (rem <= p.max_dt) {
+          cap = static_cast<T>(rem);
+          by_time = true;
+        }
+      }
+      const T hh = minT(h, cap);
+      ++att;
+      T xs[N3], vn[N3];
+      // Stages 2..6: positions x + h (c_s v + h sum_l aa[s][l] k_l), then a(x_s).
+      static_for<1, 6>([&](auto Sc) {
+        constexpr int s = decltype(Sc)::value;
+        constexpr T cs = static_cast<T>(kDP5.c[s]);
+#pragma unroll
+        for (int q = 0; q < N3; ++q) {
+          const T acc = stage_sum<T, s>(k, q, CoefAA<s>{});
+          xs[q] = fmaT(hh, fmaT(cs, v[q], hh * acc), x[q]);
+        }
+        T unused;
+        lane_force<T, N, false>(xs, k[s], unused, mc);
+      });
+      // Stage 7 (FSAL): the 5th-order solution; a(x_new) is the next k1.
+#pragma unroll
+      for (int q = 0; q < N3; ++q) {
+        const T ax = stage_sum<T, 6>(k, q, CoefAA<6>{});
+        const T av = stage_sum<T, 6>(k, q, CoefB{});
+        xs[q] = fmaT(hh, fmaT(hh, ax, v[q]), x[q]);
+        vn[q] = fmaT(hh, av, v[q]);
+      }
+      {
+        T unused;
+        lane_force<T, N, false>(xs, k[6], unused, mc);
+      }
+      // Embedded error in the RMS norm, scale = atol + rtol max(|y_old|, |y_new|).
+      const T hh2 = hh * hh;
+      T exs = T(0), evs = T(0);
+#pragma unroll
+      for (int q = 0; q < N3; ++q) {
+        const T ex = stage_sum<T, 7>(k, q, CoefEA{});
+        const T ev = stage_sum<T, 7>(k, q, CoefE{});
+        const T scx = fmaT(rtol, maxT(absT(x[q]), absT(xs[q])), atol);
+        const T scv = fmaT(rtol, maxT(absT(v[q]), absT(vn[q])), atol);
+        const T rx = (hh2 * ex) * rcpT(scx);
+        const T rv = (hh * ev) * rcpT(scv);
+        exs = fmaT(rx, rx, exs);
+        evs = fmaT(rv, rv, evs);
+      }
+      const T en = sqrtT((exs + evs) * inv6n);
+      if (en <= T(1)) {
+        bool settled = true;
+        if (p.stop_rule == kStopMomentumOrTmax) {
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            const T dx = vn[3 * i] - v[3 * i], dy = vn[3 * i + 1] - v[3 * i + 1],
+                    dz = vn[3 * i + 2] - v[3 * i + 2];
+            const T num = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
+            const T den = fmaT(vn[3 * i + 2], vn[3 * i + 2],
+                               fmaT(vn[3 * i + 1], vn[3 * i + 1], vn[3 * i] * vn[3 * i]));
+            settled = settled && (num < tol2 * den);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < N3; ++q) {
+          x[q] = xs[q];
+          v[q] = vn[q];
+          k[0][q] = k[6][q];
+        }
+        T f = (en == T(0)) ? T(10) : minT(T(10), T(0.9) * powT(en, T(-0.2)));
+        if (rej_once) f = minT(f, T(1));
+        h = minT(hh * f, cap);
+        rej_once = false;
+        nrej = 0;
+        if (p.trace_h != nullptr && steps < p.trace_cap)
+          p.trace_h[sim * p.trace_cap + steps] = static_cast<double>(hh);
+        ++steps;
+        ++tot_acc;
+        const bool final_step = by_time && hh >= cap;
+        t = final_step ? p.t_max : t + static_cast<double>(hh);
+        if (p.stop_rule == kStopPE) {
+          if (static_cast<double>(lane_pe<T, N>(x, mc)) <= pe_stop) status = kSimOkPE;
+        } else if (settled) {
+          status = kSimOkMomentum;
+        } else if (final_step) {
+          status = kSimOkTmax;
+        }
+        if (status == kRunning && steps >= p.max_steps) status = kSimNotConverged;
+      } else {
+        const T f = maxT(T(0.2), T(0.9) * powT(en, T(-0.2)));
+        h = hh * f;
+        rej_once = true;
+        if (++nrej > p.max_rej) status = (en != en) ? kSimNonfinite : kSimRejectLimit;
+      }
+    }
+
+    // ---- finalize lanes that stopped: redistribution and momenta in FP64 ----
+    if (active && status != kRunning) {
+      double ke = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+        ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
+      }
+      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0 / ke) : 1.0;
+      double pe_end = 0.0;
+      static_for<0, N>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        static_for<i + 1, N>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
+          const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
+          const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
+          pe_end += mc.pcd[pair_index<N>(i, j)] / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+        });
+      });
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double ms = mc.md[i] * s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double pm = ms * static_cast<double>(v[3 * i + c]);
+          p.mom[sim * N3 + 3 * i + c] = pm;
+          p.mom32[sim * N3 + 3 * i + c] = static_cast<float>(pm);
+        }
+      }
+      p.steps[sim] = steps;
+      p.attempts[sim] = att;
+      p.status[sim] = static_cast<uint8_t>(status);
+      p.t_final[sim] = t;
+      p.eerr[sim] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+      if (p.trace_h != nullptr)
+        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[sim * p.trace_cap + q] = 0.0;
+      tot_att += att;
+      active = false;
+    }
+  }
+
+  // ---- device totals: one atomic per wave ----
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    tot_att += __shfl_xor(tot_att, off);
+    tot_acc += __shfl_xor(tot_acc, off);
+  }
+  if (lane == 0) {
+    atomicAdd(p.ctr + 1, static_cast<unsigned long long>(tot_att));
+    atomicAdd(p.ctr + 2, static_cast<unsigned long long>(tot_acc));
+    atomicAdd(p.ctr + 3, static_cast<unsigned long long>(refills));
+  }
+}
+
+// ---------------------------------------------------------------- sampler ----
+
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
+    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    const uint32_t n0 = hi1 ^ c[1] ^ k0;
+    const uint32_t n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0;
+    c[1] = lo1;
+    c[2] = n2;
+    c[3] = lo0;
+  }
+}
+
+struct SampleArgs {
+  double* geom;        // [sim][n][3]
+  uint8_t* status;     // 0 ok / kSimSamplerFail
+  uint64_t first_index;
+  uint64_t n_sims;
+  uint32_t key0, key1;
+  float R, twoR, R2, min2;
+  uint32_t max_attempts;
+  int32_t n_atoms;
+};
+
+// One thread per simulation.  Candidate j of global sim g draws Philox4x32-10 on
+// counter (j, lo32(g), hi32(g), 0): 24-bit uniforms -> fma(u, 2R, -R); kept if
+// inside the ball and >= min_sep from every atom already placed
+// (src/sampler.cpp:28-57 placement rule).
+template <int N>
+__global__ void __launch_bounds__(256) sample_sphere(const SampleArgs a) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; s < a.n_sims;
+       s += stride) {
+    const uint64_t g = a.first_index + s;
+    uint32_t cand = 0;
+    float P[3 * N];
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (!ok) break;
+      uint32_t tries = 0;
+      for (;;) {
+        float c[3];
+        bool in_ball = false;
+        for (int cube = 0; cube < 4096; ++cube) {
+          uint32_t r[4] = {cand++, static_cast<uint32_t>(g), static_cast<uint32_t>(g >> 32), 0u};
+          philox4x32_10(r, a.key0, a.key1);
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            const float u = static_cast<float>(r[q] >> 8) * 0x1.0p-24f;
+            c[q] = __builtin_fmaf(u, a.twoR, -a.R);
+          }
+          const float d = __builtin_fmaf(c[2], c[2], __builtin_fmaf(c[1], c[1], c[0] * c[0]));
+          if (d <= a.R2) {
+            in_ball = true;
+            break;
+          }
+        }
+        if (!in_ball) {
+          ok = false;
+          break;
+        }
+        bool clears = true;
+#pragma unroll
+        for (int j = 0; j < i; ++j) {
+          const float dx = c[0] - P[3 * j], dy = c[1] - P[3 * j + 1], dz = c[2] - P[3 * j + 2];
+          const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+          clears = clears && !(d2 < a.min2);
+        }
+        if (clears) {
+          P[3 * i] = c[0];
+          P[3 * i + 1] = c[1];
+          P[3 * i + 2] = c[2];
+          break;
+        }
+        if (++tries >= a.max_attempts) {
+          ok = false;
+          break;
+        }
+      }
+    }
+    double* out = a.geom + s * (3 * N);
+#pragma unroll
+    for (int q = 0; q < 3 * N; ++q) out[q] = ok ? static_cast<double>(P[q]) : 0.0;
+    a.status[s] = ok ? 0 : kSimSamplerFail;
+  }
+}
+
+// ------------------------------------------------- reference-layout kernels ----
+// batched::accelerations / potential_energy (batched_force.hpp:31-110) on the
+// reference's SoA layout p[i*k + lane], one thread per lane, with the
+// reference's IEEE 1/sqrt and division so results match it bit for bit.
+template <int N>
+__global__ void __launch_bounds__(256) soa_accel(uint64_t k, const float* px, const float* py,
+                                                 const float* pz, const float* pc, const float* im,
+                                                 float* ax, float* ay, float* az) {
+  const uint64_t l = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (l >= k) return;
+  float x[N], y[N], z[N], fx[N], fy[N], fz[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    x[i] = px[i * k + l];
+    y[i] = py[i * k + l];
+    z[i] = pz[i * k + l];
+    fx[i] = fy[i] = fz[i] = 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      const float rx = x[i] - x[j], ry = y[i] - y[j], rz = z[i] - z[j];
+      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+      const float inv = 1.0f / __builtin_sqrtf(d2);
+      const float inv3 = inv / d2;
+      const float s = pc[i * N + j] * inv3;
+      const float gx = rx * s, gy = ry * s, gz = rz * s;
+      fx[i] += gx;
+      fy[i] += gy;
+      fz[i] += gz;
+      fx[j] -= gx;
+      fy[j] -= gy;
+      fz[j] -= gz;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    ax[i * k + l] = fx[i] * im[i];
+    ay[i * k + l] = fy[i] * im[i];
+    az[i * k + l] = fz[i] * im[i];
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(256) soa_pe(uint64_t k, const float* px, const float* py,
+                                              const float* pz, const float* pc, float* pe) {
+  const uint64_t l = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (l >= k) return;
+  float u = 0.0f;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      const float rx = px[i * k + l] - px[j * k + l];
+      const float ry = py[i * k + l] - py[j * k + l];
+      const float rz = pz[i * k + l] - pz[j * k + l];
+      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+      u = __builtin_fmaf(pc[i * N + j], 1.0f / __builtin_sqrtf(d2), u);
+    }
+  pe[l] = u;
+}
+
+__global__ void zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+}  // namespace ce

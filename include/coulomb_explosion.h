@@ -1,0 +1,209 @@
+/* coulomb_explosion.h -- C-ABI of the MI355X (gfx950) Coulomb-explosion engine.
+ *
+ * This is the drop-in boundary for the reference's batched f32 path: the
+ * reference's BatchedRK45<float> (include/coulomb/batched_integrator.hpp:62-191)
+ * with its sampler (include/coulomb/sampler.hpp:46-72) and its batched kernels
+ * (include/coulomb/batched_force.hpp:31-110).  The reference has no FFI; its
+ * C++ interface maps onto these entry points as noted per function, and
+ * include/coulomb_gpu.hpp wraps them back into that C++ interface.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only; every entry point returns int (CE_OK = 0,
+ *     negative = error) and never throws.  ce_last_error() gives the message of
+ *     the last failure on the calling thread.
+ *   - CE_ERR_INVALID_ARGUMENT corresponds to the reference's
+ *     std::invalid_argument, CE_ERR_RUNTIME to std::runtime_error.
+ *   - An engine owns its device buffers and one HIP stream.  It is
+ *     thread-compatible (one engine per host thread), like the reference
+ *     instance.  Host pointers are host memory unless a name says _device.
+ *   - Positions are atomic units (bohr), masses electron masses, time a.u.
+ */
+#ifndef COULOMB_EXPLOSION_H
+#define COULOMB_EXPLOSION_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CE_MAX_ATOMS 10
+
+enum ce_error {
+  CE_OK = 0,
+  CE_ERR_INVALID_ARGUMENT = -1,
+  CE_ERR_RUNTIME = -2,
+  CE_ERR_HIP = -3,
+  CE_ERR_NO_DEVICE = -4
+};
+
+/* Termination rule.  CE_STOP_PE_FRACTION is the reference's
+ * (driver.hpp:28-35, batched_integrator.hpp:149-169): stop once
+ * PE <= pe_stop_fraction * PE0.  CE_STOP_MOMENTUM_OR_TMAX stops when every
+ * atom's momentum changed by less than momentum_tol (relative) over the last
+ * accepted step, or when the FP64 time reaches t_max (the last step is clamped
+ * to land on t_max). */
+enum ce_stop_rule { CE_STOP_PE_FRACTION = 0, CE_STOP_MOMENTUM_OR_TMAX = 1 };
+/* Initial step: the reference's Hairer heuristic (integrators.cpp:197-250)
+ * or a fixed h0. */
+enum ce_h0_mode { CE_H0_HAIRER = 0, CE_H0_FIXED = 1 };
+/* Arithmetic of the integration: f32 state (production) or the f64 path used
+ * as the on-GPU accuracy oracle. */
+enum ce_precision { CE_PRECISION_F32 = 0, CE_PRECISION_F64 = 1 };
+
+/* Per-simulation status byte. */
+enum ce_sim_status {
+  CE_SIM_OK_PE = 0,         /* PE fraction reached */
+  CE_SIM_OK_MOMENTUM = 1,   /* momentum settled */
+  CE_SIM_OK_TMAX = 2,       /* reached t_max */
+  CE_SIM_NOT_CONVERGED = 3, /* max_steps hit first (reference: converged=0) */
+  CE_SIM_REJECT_LIMIT = 4,  /* > max_rejections in one step (reference throws) */
+  CE_SIM_NONFINITE = 5,     /* error norm NaN through the rejection limit */
+  CE_SIM_SAMPLER_FAIL = 6   /* device sampler could not place the atoms */
+};
+
+/* Molecule: replaces the reference's Molecule (molecule.hpp:30-35) and the
+ * BatchedRK45 constructor's pair_const / inv_mass set-up
+ * (batched_integrator.hpp:71-106).  1 <= n_atoms <= CE_MAX_ATOMS. */
+typedef struct {
+  int32_t n_atoms;
+  int32_t pad_;
+  const double* masses;  /* n_atoms, electron masses */
+  const double* charges; /* n_atoms */
+  double coulomb_k;
+} ce_molecule_desc;
+
+/* Tolerances: BatchedRK45(mol, k, lanes, rtol, atol) (batched_integrator.hpp:71). */
+typedef struct {
+  double rtol;
+  double atol;
+} ce_tolerances;
+
+/* Run configuration: the reference's RunConfig (driver.hpp:28-50) plus the
+ * engine's added fields.  ce_run_config_default() fills the reference defaults
+ * (pe_stop_fraction 1e-9, redistribute on, max_dt 1e12, max_steps 1e6, Hairer
+ * start, 100 rejections, f32). */
+typedef struct {
+  double pe_stop_fraction;
+  double max_dt;
+  double t_max;
+  double momentum_tol;
+  double h0;
+  uint64_t max_steps;
+  int32_t stop_rule;
+  int32_t redistribute_energy;
+  int32_t h0_mode;
+  int32_t max_rejections;
+  int32_t precision;
+  int32_t pad_;
+} ce_run_config;
+
+/* Result buffers, caller-allocated host memory; a NULL field is not copied.
+ * momenta mirrors BatchedRK45::Result::momenta [(sim*n + i)*3 + c]
+ * (batched_integrator.hpp:64-69); momenta_f32 is the same in float (the
+ * dataset record's final momenta). */
+typedef struct {
+  double* momenta;
+  float* momenta_f32;
+  uint32_t* steps;     /* accepted steps of the sim */
+  uint32_t* attempts;  /* attempted steps (accepted + rejected) */
+  uint8_t* status;     /* ce_sim_status */
+  double* t_final;     /* FP64 time at the stop */
+  double* energy_err;  /* (KE + PE - E0)/E0 at the stop, FP64, before redistribution */
+  double* trace_h;     /* [sim][trace_cap] accepted step sizes (0 past the end) */
+  uint32_t trace_cap;  /* 0 = no trace; must match ce_engine_enable_trace */
+  uint32_t pad_;
+} ce_results;
+
+/* Totals of the last run, measured on the device. */
+typedef struct {
+  double kernel_ms;          /* integration kernel, HIP events on the engine stream */
+  double sampler_ms;         /* last device-sampler kernel */
+  uint64_t sims;
+  uint64_t attempted_steps;  /* summed over sims */
+  uint64_t accepted_steps;
+  uint64_t refills;          /* wave refill events */
+  int32_t grid_blocks;
+  int32_t block_threads;
+  int32_t n_cus;
+  int32_t pad_;
+} ce_run_stats;
+
+/* Launch shape: 0 fields mean "engine default".  refill_threshold = idle lanes a
+ * wave waits for before it pulls new sims (1..64); chunk = sims a wave claims
+ * from the device work queue per atomic. */
+typedef struct {
+  int32_t block_threads;     /* multiple of 64, <= 1024 */
+  int32_t blocks_per_cu;     /* resident blocks per CU to launch */
+  int32_t refill_threshold;
+  int32_t chunk;
+} ce_launch_config;
+
+typedef struct ce_engine ce_engine;
+
+const char* ce_last_error(void);
+int ce_device_count(int* count);
+void ce_run_config_default(ce_run_config* cfg);
+
+/* Replaces BatchedRK45(const Molecule&, Real coulomb_k, size_t lanes, T rtol,
+ * T atol) (batched_integrator.hpp:71-106).  `capacity` is the most sims one run
+ * holds (the reference's `lanes` is the SIMD width; here it is the batch). */
+int ce_engine_create(const ce_molecule_desc* mol, const ce_tolerances* tol, size_t capacity,
+                     ce_engine** out);
+int ce_engine_destroy(ce_engine* eng);
+
+/* Replaces set_lane_geometry(lane, positions) (batched_integrator.hpp:112-121)
+ * for sims [0, n_sims): pos is AoS doubles [(sim*n + i)*3 + c], velocities start
+ * at rest.  The f32 path narrows to float on the device as the reference does. */
+int ce_engine_set_geometries(ce_engine* eng, const double* pos, size_t n_sims);
+/* Device-side sampler (no reference counterpart; replaces the host loop of
+ * UniformSphereSampler::sample, sampler.cpp:28-57, by a counter-based one):
+ * sims [0, n_sims) get global indices first_index + s, so results do not depend
+ * on how a job is split.  Sims that cannot be placed get CE_SIM_SAMPLER_FAIL. */
+int ce_engine_sample(ce_engine* eng, uint64_t seed, uint64_t first_index, size_t n_sims,
+                     double radius, double min_sep, uint32_t max_attempts);
+/* Copies the current geometries back (AoS doubles). */
+int ce_engine_get_geometries(ce_engine* eng, double* pos, size_t n_sims);
+
+/* Replaces BatchedRK45::run(const RunConfig&) (batched_integrator.hpp:123-191):
+ * integrates every loaded sim to its stop and copies the requested results
+ * into `out` (may be NULL: results stay on the device). */
+int ce_engine_run(ce_engine* eng, const ce_run_config* cfg, ce_results* out);
+/* Copies the last run's results (same buffers as ce_engine_run). */
+int ce_engine_download(ce_engine* eng, ce_results* out);
+int ce_engine_stats(const ce_engine* eng, ce_run_stats* out);
+int ce_engine_set_launch(ce_engine* eng, const ce_launch_config* lc);
+int ce_engine_get_launch(const ce_engine* eng, ce_launch_config* lc);
+/* Records the first trace_cap accepted step sizes of every sim on the next
+ * runs (the reference's StepObserver hook, driver.hpp:12-18); 0 disables. */
+int ce_engine_enable_trace(ce_engine* eng, uint32_t trace_cap);
+
+/* Host sampler with the reference's exact stream: UniformSphereSampler(Options,
+ * seed) (sampler.hpp:46-72, sampler.cpp:7-57) -- std::mt19937_64 feeding
+ * std::uniform_real_distribution<double>(-R, R), cube rejection, sequential
+ * min-separation placement.  Stateful like the reference: successive calls
+ * continue the stream.  Create fails with CE_ERR_INVALID_ARGUMENT for
+ * radius <= 0 or min_separation < 0; sample fails with CE_ERR_RUNTIME when an
+ * atom cannot be placed in max_attempts tries (sims before it are written). */
+typedef struct ce_host_sampler ce_host_sampler;
+int ce_host_sampler_create(uint64_t seed, double radius, double min_separation,
+                           uint64_t max_attempts, ce_host_sampler** out);
+int ce_host_sampler_reseed(ce_host_sampler* s, uint64_t seed);
+int ce_host_sampler_sample(ce_host_sampler* s, int32_t n_atoms, size_t n_sims, double* pos_out);
+int ce_host_sampler_destroy(ce_host_sampler* s);
+
+/* Replaces batched::accelerations<float> (batched_force.hpp:31-84) with the same
+ * SoA layout p[i*k + lane], pair_const[i*n + j], inv_mass[i]; host pointers. */
+int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const float* py,
+                                 const float* pz, const float* pair_const, const float* inv_mass,
+                                 float* ax, float* ay, float* az);
+/* Replaces batched::potential_energy<float> (batched_force.hpp:89-110). */
+int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const float* py,
+                                    const float* pz, const float* pair_const, float* pe);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COULOMB_EXPLOSION_H */

@@ -1,0 +1,509 @@
+// CPU oracle, per-lane formulation -- TEST INFRASTRUCTURE ONLY (see ce_oracle.h).
+//
+// The GPU engine integrates one simulation per lane with its own step size
+// (no shared dt, converged lanes refilled).  Per lane it is the reference's
+// DormandPrince45 (src/integrators.cpp:59-250) -- same tableau, FSAL, RMS error
+// norm, controller and Hairer start -- with one algebraic rearrangement that
+// halves the stage storage: because dx/dt = v, the stage velocities are
+//   v_s = v + h * sum_l A[s][l] k_l            (k_l = stage accelerations)
+// and the stage positions the reference builds from them collapse to
+//   x_s = x + h * (c_s v + h * sum_l (A A)[s][l] k_l),
+// while the position half of the embedded error is h^2 * sum_l (E A)[l] k_l
+// (sum_s E_s = 0).  Only the seven k_l are stored.  This file is the CPU
+// statement of that arithmetic, operation for operation, so the kernel can be
+// checked tightly against it; the reference forms in scalar.cpp / batched.hpp
+// pin it to the reference at the engine's accuracy bar.
+//
+// Also here: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random
+// numbers: as easy as 1, 2, 3", SC'11) and the engine's counter-based
+// uniform-sphere sampler built on it (cube rejection + sequential
+// min-separation placement, the placement rule of src/sampler.cpp:28-57).
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "ce_oracle.h"
+#include "oracle_common.hpp"
+
+namespace ceo {
+namespace {
+
+struct LaneTableD {
+  double c[7], aa[7][7], b[7], e[7], ea[7];
+};
+
+const LaneTableD& lane_table() {
+  static const LaneTableD t = [] {
+    const DP5Table& d = dp5_table();
+    LaneTableD r{};
+    const double c[7] = {0.0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
+    for (int s = 0; s < 7; ++s) r.c[s] = c[s];
+    for (int s = 0; s < 7; ++s)
+      for (int l = 0; l < 7; ++l) {
+        double acc = 0.0;
+        for (int j = l + 1; j < s; ++j) acc += d.a[s][j] * d.a[j][l];
+        r.aa[s][l] = acc;
+      }
+    for (int l = 0; l < 7; ++l) {
+      r.b[l] = d.a[6][l];
+      r.e[l] = d.e[l];
+      double acc = 0.0;
+      for (int s = l + 1; s < 7; ++s) acc += d.e[s] * d.a[s][l];
+      r.ea[l] = acc;
+    }
+    return r;
+  }();
+  return t;
+}
+
+template <class T>
+struct LaneCoef {
+  T c[7], aa[7][7], b[7], e[7], ea[7];
+};
+
+template <class T>
+const LaneCoef<T>& lane_coef() {
+  static const LaneCoef<T> k = [] {
+    const LaneTableD& d = lane_table();
+    LaneCoef<T> r{};
+    for (int s = 0; s < 7; ++s) {
+      r.c[s] = static_cast<T>(d.c[s]);
+      r.b[s] = static_cast<T>(d.b[s]);
+      r.e[s] = static_cast<T>(d.e[s]);
+      r.ea[s] = static_cast<T>(d.ea[s]);
+      for (int l = 0; l < 7; ++l) r.aa[s][l] = static_cast<T>(d.aa[s][l]);
+    }
+    return r;
+  }();
+  return k;
+}
+
+constexpr int kMaxN = 16;
+
+template <class T>
+struct LaneMol {
+  int n;
+  T pc[kMaxN][kMaxN];  // k q_i q_j narrowed from double
+  T im[kMaxN];         // 1/m_i narrowed from double
+  double pcd[kMaxN][kMaxN];
+  double md[kMaxN];
+};
+
+template <class T>
+LaneMol<T> lane_mol(const ceo_molecule* m) {
+  LaneMol<T> r{};
+  r.n = m->n;
+  for (int i = 0; i < m->n; ++i) {
+    r.md[i] = m->masses[i];
+    r.im[i] = static_cast<T>(1.0 / m->masses[i]);
+    for (int j = 0; j < m->n; ++j) {
+      r.pcd[i][j] = m->coulomb_k * m->charges[i] * m->charges[j];
+      r.pc[i][j] = static_cast<T>(r.pcd[i][j]);
+    }
+  }
+  return r;
+}
+
+// Accelerations (and optionally PE) of one lane: x, a are [3n] (atom-major).
+template <class T>
+void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe) {
+  const int n = m.n;
+  for (int q = 0; q < 3 * n; ++q) a[q] = T(0);
+  T u = T(0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const T rx = x[3 * i] - x[3 * j];
+      const T ry = x[3 * i + 1] - x[3 * j + 1];
+      const T rz = x[3 * i + 2] - x[3 * j + 2];
+      const T d2 = std::fma(rx, rx, std::fma(ry, ry, rz * rz));
+      const T inv = T(1) / std::sqrt(d2);
+      const T inv3 = (inv * inv) * inv;
+      const T s = m.pc[i][j] * inv3;
+      if (pe) u = std::fma(m.pc[i][j], inv, u);
+      const T fx = rx * s, fy = ry * s, fz = rz * s;
+      a[3 * i] += fx;
+      a[3 * i + 1] += fy;
+      a[3 * i + 2] += fz;
+      a[3 * j] -= fx;
+      a[3 * j + 1] -= fy;
+      a[3 * j + 2] -= fz;
+    }
+  for (int i = 0; i < n; ++i) {
+    a[3 * i] *= m.im[i];
+    a[3 * i + 1] *= m.im[i];
+    a[3 * i + 2] *= m.im[i];
+  }
+  if (pe) *pe = u;
+}
+
+template <class T>
+T lane_pe(const LaneMol<T>& m, const T* x) {
+  T u = T(0);
+  for (int i = 0; i < m.n; ++i)
+    for (int j = i + 1; j < m.n; ++j) {
+      const T rx = x[3 * i] - x[3 * j];
+      const T ry = x[3 * i + 1] - x[3 * j + 1];
+      const T rz = x[3 * i + 2] - x[3 * j + 2];
+      const T d2 = std::fma(rx, rx, std::fma(ry, ry, rz * rz));
+      u = std::fma(m.pc[i][j], T(1) / std::sqrt(d2), u);
+    }
+  return u;
+}
+
+// Hairer / scipy initial step (src/integrators.cpp:197-250) for one lane, in the
+// scalar reference's accumulation order; needs k0 = a(x0) and v.
+template <class T>
+T lane_hairer(const LaneMol<T>& m, const T* x, const T* v, const T* a0, T rtol, T atol,
+              T max_dt) {
+  const int n = m.n;
+  T d0s = T(0), d1s = T(0);
+  for (int q = 0; q < 3 * n; ++q) {
+    const T sx = atol + rtol * std::abs(x[q]);
+    const T sv = atol + rtol * std::abs(v[q]);
+    T r = x[q] / sx;
+    d0s += r * r;
+    r = v[q] / sv;
+    d0s += r * r;
+    r = v[q] / sx;
+    d1s += r * r;
+    r = a0[q] / sv;
+    d1s += r * r;
+  }
+  const T den = static_cast<T>(6 * n);
+  const T d0 = std::sqrt(d0s / den);
+  const T d1 = std::sqrt(d1s / den);
+  const T h0 = (d0 < T(1e-5) || d1 < T(1e-5)) ? T(1e-6) : T(0.01) * d0 / d1;
+  T xs[3 * kMaxN], a1[3 * kMaxN];
+  for (int q = 0; q < 3 * n; ++q) xs[q] = x[q] + v[q] * h0;
+  lane_force(m, xs, a1, static_cast<T*>(nullptr));
+  T d2s = T(0);
+  for (int q = 0; q < 3 * n; ++q) {
+    const T sx = atol + rtol * std::abs(x[q]);
+    const T sv = atol + rtol * std::abs(v[q]);
+    T r = (a0[q] * h0) / sx;
+    d2s += r * r;
+    r = (a1[q] - a0[q]) / sv;
+    d2s += r * r;
+  }
+  const T d2 = std::sqrt(d2s / den) / h0;
+  const T h1 = (d1 <= T(1e-15) && d2 <= T(1e-15)) ? std::max(T(1e-6), h0 * T(1e-3))
+                                                   : std::pow(T(0.01) / std::max(d1, d2), T(0.2));
+  return std::min(std::min(T(100) * h0, h1), max_dt);
+}
+
+template <class T>
+void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, size_t sim,
+                  ceo_results* out) {
+  const LaneCoef<T>& K = lane_coef<T>();
+  const int n = m.n, n3 = 3 * m.n;
+  const T rtol = static_cast<T>(cfg->rtol), atol = static_cast<T>(cfg->atol);
+  const T max_dt = static_cast<T>(cfg->max_dt);
+  T x[3 * kMaxN], v[3 * kMaxN], k[7][3 * kMaxN], xn[3 * kMaxN], vn[3 * kMaxN];
+  for (int q = 0; q < n3; ++q) {
+    x[q] = static_cast<T>(p0[q]);
+    v[q] = T(0);
+  }
+  T pe0;
+  lane_force(m, x, k[0], &pe0);
+  const double e0 = static_cast<double>(pe0);
+  const double pe_stop = cfg->pe_stop_fraction * e0;
+  const T tol2 = static_cast<T>(cfg->momentum_tol * cfg->momentum_tol);
+  uint32_t steps = 0, attempts = 0;
+  int nrej = 0;
+  bool rej_once = false;
+  uint8_t status = 255;
+  double t = 0.0;
+  std::vector<double> trace;
+  T h;
+  if (cfg->stop_rule == CEO_STOP_PE_FRACTION && e0 <= pe_stop) {
+    status = CEO_OK_PE;
+    h = T(0);
+  } else if (cfg->max_steps == 0) {
+    status = CEO_NOT_CONVERGED;
+    h = T(0);
+  } else {
+    h = (cfg->h0_mode == CEO_H0_FIXED) ? static_cast<T>(cfg->h0)
+                                       : lane_hairer(m, x, v, k[0], rtol, atol, max_dt);
+  }
+  const T inv6n = T(1) / static_cast<T>(6 * n);
+  while (status == 255) {
+    T cap = max_dt;
+    bool by_time = false;
+    if (cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX) {
+      const double rem = cfg->t_max - t;
+      if (rem <= cfg->max_dt) {
+        cap = static_cast<T>(rem);
+        by_time = true;
+      }
+    }
+    const T hh = std::min(h, cap);
+    ++attempts;
+    // Stages 2..6: positions only.
+    for (int s = 1; s < 6; ++s) {
+      for (int q = 0; q < n3; ++q) {
+        T acc = T(0);
+        bool first = true;
+        for (int l = 0; l < s; ++l) {
+          if (K.aa[s][l] == T(0)) continue;
+          acc = first ? K.aa[s][l] * k[l][q] : std::fma(K.aa[s][l], k[l][q], acc);
+          first = false;
+        }
+        xn[q] = std::fma(hh, std::fma(K.c[s], v[q], hh * acc), x[q]);
+      }
+      lane_force(m, xn, k[s], static_cast<T*>(nullptr));
+    }
+    // Stage 7 (FSAL): the 5th-order solution itself.
+    for (int q = 0; q < n3; ++q) {
+      T ax = T(0), av = T(0);
+      bool fx = true, fv = true;
+      for (int l = 0; l < 6; ++l) {
+        if (K.aa[6][l] != T(0)) {
+          ax = fx ? K.aa[6][l] * k[l][q] : std::fma(K.aa[6][l], k[l][q], ax);
+          fx = false;
+        }
+        if (K.b[l] != T(0)) {
+          av = fv ? K.b[l] * k[l][q] : std::fma(K.b[l], k[l][q], av);
+          fv = false;
+        }
+      }
+      xn[q] = std::fma(hh, std::fma(hh, ax, v[q]), x[q]);
+      vn[q] = std::fma(hh, av, v[q]);
+    }
+    lane_force(m, xn, k[6], static_cast<T*>(nullptr));
+    // Embedded error, RMS over the 6n components.
+    const T hh2 = hh * hh;
+    T ex_sum = T(0), ev_sum = T(0);
+    for (int q = 0; q < n3; ++q) {
+      T ex = T(0), ev = T(0);
+      bool f1 = true, f2 = true;
+      for (int l = 0; l < 7; ++l) {
+        if (K.ea[l] != T(0)) {
+          ex = f1 ? K.ea[l] * k[l][q] : std::fma(K.ea[l], k[l][q], ex);
+          f1 = false;
+        }
+        if (K.e[l] != T(0)) {
+          ev = f2 ? K.e[l] * k[l][q] : std::fma(K.e[l], k[l][q], ev);
+          f2 = false;
+        }
+      }
+      const T scx = std::fma(rtol, std::max(std::abs(x[q]), std::abs(xn[q])), atol);
+      const T scv = std::fma(rtol, std::max(std::abs(v[q]), std::abs(vn[q])), atol);
+      const T rx = (hh2 * ex) * (T(1) / scx);
+      const T rv = (hh * ev) * (T(1) / scv);
+      ex_sum = std::fma(rx, rx, ex_sum);
+      ev_sum = std::fma(rv, rv, ev_sum);
+    }
+    const T en = std::sqrt((ex_sum + ev_sum) * inv6n);
+    if (en <= T(1)) {
+      bool settled = true;
+      if (cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX) {
+        for (int i = 0; i < n && settled; ++i) {
+          const T dx = vn[3 * i] - v[3 * i], dy = vn[3 * i + 1] - v[3 * i + 1],
+                  dz = vn[3 * i + 2] - v[3 * i + 2];
+          const T num = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+          const T den = std::fma(vn[3 * i + 2], vn[3 * i + 2],
+                                 std::fma(vn[3 * i + 1], vn[3 * i + 1], vn[3 * i] * vn[3 * i]));
+          settled = num < tol2 * den;
+        }
+      }
+      for (int q = 0; q < n3; ++q) {
+        x[q] = xn[q];
+        v[q] = vn[q];
+        k[0][q] = k[6][q];
+      }
+      T f = (en == T(0)) ? T(10) : std::min(T(10), T(0.9) * std::pow(en, T(-0.2)));
+      if (rej_once) f = std::min(f, T(1));
+      h = std::min(hh * f, cap);
+      rej_once = false;
+      nrej = 0;
+      ++steps;
+      trace.push_back(static_cast<double>(hh));
+      const bool final_step = by_time && hh >= cap;
+      t = final_step ? cfg->t_max : t + static_cast<double>(hh);
+      if (cfg->stop_rule == CEO_STOP_PE_FRACTION) {
+        if (static_cast<double>(lane_pe(m, x)) <= pe_stop) status = CEO_OK_PE;
+      } else if (settled) {
+        status = CEO_OK_MOMENTUM;
+      } else if (final_step) {
+        status = CEO_OK_TMAX;
+      }
+      if (status == 255 && steps >= cfg->max_steps) status = CEO_NOT_CONVERGED;
+    } else {
+      const T f = std::max(T(0.2), T(0.9) * std::pow(en, T(-0.2)));
+      h = hh * f;
+      rej_once = true;
+      if (++nrej > cfg->max_rejections) status = std::isnan(en) ? CEO_NONFINITE : CEO_REJECT_LIMIT;
+    }
+  }
+  // Finalize in double: redistribution s = sqrt(E0/KE), p = m s v.
+  double ke = 0.0;
+  for (int i = 0; i < n; ++i) {
+    const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+    ke += 0.5 * m.md[i] * (vx * vx + vy * vy + vz * vz);
+  }
+  double s = 1.0;
+  if (cfg->redistribute && ke > 0.0) s = std::sqrt(e0 / ke);
+  double pe_end = 0.0;
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j) {
+      const double rx = double(x[3 * i]) - double(x[3 * j]);
+      const double ry = double(x[3 * i + 1]) - double(x[3 * j + 1]);
+      const double rz = double(x[3 * i + 2]) - double(x[3 * j + 2]);
+      pe_end += m.pcd[i][j] / std::sqrt(rx * rx + ry * ry + rz * rz);
+    }
+  for (int i = 0; i < n; ++i) {
+    const double ms = m.md[i] * s;
+    double* p = out->momenta + (sim * n + i) * 3;
+    p[0] = ms * double(v[3 * i]);
+    p[1] = ms * double(v[3 * i + 1]);
+    p[2] = ms * double(v[3 * i + 2]);
+  }
+  out->steps[sim] = steps;
+  out->status[sim] = status;
+  if (out->attempts) out->attempts[sim] = attempts;
+  if (out->t_final) out->t_final[sim] = t;
+  if (out->energy_err) out->energy_err[sim] = (ke + pe_end - e0) / e0;
+  if (out->trace_h)
+    for (uint32_t q = 0; q < out->trace_cap; ++q)
+      out->trace_h[sim * out->trace_cap + q] = q < trace.size() ? trace[q] : 0.0;
+}
+
+template <class T>
+int lane_run(const ceo_molecule* cm, const ceo_config* cfg, const double* pos, size_t n_sims,
+             ceo_results* out) {
+  if (cm->n < 1 || cm->n > kMaxN) return -1;
+  const LaneMol<T> m = lane_mol<T>(cm);
+  for (size_t sim = 0; sim < n_sims; ++sim)
+    lane_run_one<T>(m, cfg, pos + sim * cm->n * 3, sim, out);
+  return 0;
+}
+
+inline void philox_round(uint32_t c[4], const uint32_t k[2]) {
+  const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+  const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+  const uint32_t hi0 = static_cast<uint32_t>(p0 >> 32), lo0 = static_cast<uint32_t>(p0);
+  const uint32_t hi1 = static_cast<uint32_t>(p1 >> 32), lo1 = static_cast<uint32_t>(p1);
+  const uint32_t n0 = hi1 ^ c[1] ^ k[0];
+  const uint32_t n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0;
+  c[1] = lo1;
+  c[2] = n2;
+  c[3] = lo0;
+}
+
+}  // namespace
+}  // namespace ceo
+
+using namespace ceo;
+
+extern "C" {
+
+int ceo_lane_run_f32(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                     size_t n_sims, ceo_results* out) {
+  return lane_run<float>(mol, cfg, pos, n_sims, out);
+}
+
+int ceo_lane_run_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                     size_t n_sims, ceo_results* out) {
+  return lane_run<double>(mol, cfg, pos, n_sims, out);
+}
+
+void ceo_lane_coefficients(double* c, double* aa, double* b, double* e, double* ea) {
+  const LaneTableD& t = lane_table();
+  for (int s = 0; s < 7; ++s) {
+    c[s] = t.c[s];
+    b[s] = t.b[s];
+    e[s] = t.e[s];
+    ea[s] = t.ea[s];
+    for (int l = 0; l < 7; ++l) aa[s * 7 + l] = t.aa[s][l];
+  }
+}
+
+void ceo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+  uint32_t k[2] = {key[0], key[1]};
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k[0] += 0x9E3779B9u;
+      k[1] += 0xBB67AE85u;
+    }
+    philox_round(c, k);
+  }
+  std::memcpy(out, c, sizeof(c));
+}
+
+// The engine's counter-based uniform-sphere sampler.  Per simulation g, candidate
+// number j uses Philox4x32-10 with counter (j, lo32(g), hi32(g), 0) and key
+// (lo32(seed), hi32(seed)); words 0..2 -> 24-bit uniforms u in [0,1) ->
+// coordinates fma(u, 2R, -R) in float.  A candidate is kept if it lies in the ball
+// (cube rejection) and clears every atom placed before it by min_sep
+// (src/sampler.cpp:28-57); an atom failing max_attempts separation tests, or a
+// candidate needing more than 4096 cube draws, marks the sim CEO_SAMPLER_FAIL.
+size_t ceo_device_sample(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
+                         double radius, double min_sep, uint32_t max_attempts, double* pos_out,
+                         uint8_t* status_out) {
+  const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
+  const float R = static_cast<float>(radius);
+  const float twoR = static_cast<float>(2.0 * radius);
+  const float R2 = R * R;
+  const float ms = static_cast<float>(min_sep);
+  const float min2 = ms * ms;
+  size_t fails = 0;
+  for (size_t s = 0; s < n_sims; ++s) {
+    const uint64_t g = first_index + s;
+    uint32_t cand = 0;
+    float P[3 * kMaxN];
+    bool ok_sim = true;
+    for (int i = 0; i < n_atoms && ok_sim; ++i) {
+      uint32_t tries = 0;
+      for (;;) {
+        float c[3];
+        bool in_ball = false;
+        for (int cube = 0; cube < 4096; ++cube) {
+          const uint32_t ctr[4] = {cand++, static_cast<uint32_t>(g), static_cast<uint32_t>(g >> 32),
+                                   0u};
+          uint32_t r[4];
+          ceo_philox4x32_10(ctr, key, r);
+          for (int q = 0; q < 3; ++q) {
+            const float u = static_cast<float>(r[q] >> 8) * 0x1.0p-24f;
+            c[q] = std::fma(u, twoR, -R);
+          }
+          const float d = std::fma(c[2], c[2], std::fma(c[1], c[1], c[0] * c[0]));
+          if (d <= R2) {
+            in_ball = true;
+            break;
+          }
+        }
+        if (!in_ball) {
+          ok_sim = false;
+          break;
+        }
+        bool clears = true;
+        for (int j = 0; j < i && clears; ++j) {
+          const float dx = c[0] - P[3 * j], dy = c[1] - P[3 * j + 1], dz = c[2] - P[3 * j + 2];
+          const float d2 = std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+          clears = !(d2 < min2);
+        }
+        if (clears) {
+          P[3 * i] = c[0];
+          P[3 * i + 1] = c[1];
+          P[3 * i + 2] = c[2];
+          break;
+        }
+        if (++tries >= max_attempts) {
+          ok_sim = false;
+          break;
+        }
+      }
+    }
+    double* out = pos_out + s * n_atoms * 3;
+    for (int q = 0; q < 3 * n_atoms; ++q) out[q] = ok_sim ? static_cast<double>(P[q]) : 0.0;
+    if (status_out) status_out[s] = ok_sim ? 0 : CEO_SAMPLER_FAIL;
+    if (!ok_sim) ++fails;
+  }
+  return fails;
+}
+
+}  // extern "C"

@@ -1,0 +1,128 @@
+"""GPU parity: the HIP engine through the C-ABI against the CPU oracle.
+
+Sizes are ones the oracle finishes in seconds.  Tolerances:
+  * f32 engine vs the oracle's f32 per-lane statement and vs the reference-form
+    engines: 1e-5 relative per-atom momentum (the engine's accuracy bar);
+  * f64 engine vs the oracle's f64 per-lane statement: 1e-12 (same arithmetic,
+    correctly rounded sqrt/div on both sides);
+  * reference-layout force / PE kernels: bit-exact.
+"""
+import numpy as np
+import pytest
+
+from conftest import CH4_CHARGES, CH4_MASSES, per_atom_rel
+
+pytestmark = pytest.mark.gpu
+
+BASELINE_RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+
+
+def ch4(ce):
+    return ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] +
+                       [ce.Atom("H", m, 1.0) for m in CH4_MASSES[1:]])
+
+
+def engine_run(ce, mol, geom, rtol, atol, **cfg):
+    n_sims = geom.shape[0]
+    eng = ce.BatchedRK45(mol, 1.0, n_sims, rtol, atol)
+    eng.set_geometries(geom)
+    res = eng.run(ce.RunConfig(**cfg))
+    return eng, res
+
+
+def test_soa_force_kernels_bit_exact(gpu, oracle):
+    """batched::accelerations / potential_energy layout, vs the oracle's lockstep
+    restatement arithmetic (fma d2, IEEE 1/sqrt then /d2)."""
+    ce = gpu
+    rng = np.random.default_rng(7)
+    for n in (2, 5, 8, 10):
+        k = 1000
+        pos = rng.uniform(-3, 3, size=(3, n, k)).astype(np.float32)
+        q = rng.integers(1, 3, size=n).astype(np.float64)
+        m = rng.uniform(1800, 30000, size=n)
+        pc = np.outer(q, q).astype(np.float32).reshape(-1)
+        im = (1.0 / m).astype(np.float32)
+        ax, ay, az = ce.batched_accelerations(n, k, pos[0].reshape(-1), pos[1].reshape(-1),
+                                              pos[2].reshape(-1), pc, im)
+        flat = [pos[c].reshape(-1) for c in range(3)]
+        pe = ce.batched_potential_energy(n, k, *flat, pc)
+        rx, ry, rz = oracle.soa_accelerations(n, k, *flat, pc, im)
+        ref_pe = oracle.soa_potential_energy(n, k, *flat, pc)
+        np.testing.assert_array_equal(ax, rx)
+        np.testing.assert_array_equal(ay, ry)
+        np.testing.assert_array_equal(az, rz)
+        np.testing.assert_array_equal(pe, ref_pe)
+
+
+def test_device_sampler_bit_exact(gpu, oracle):
+    ce = gpu
+    for n, R, ms, seed in ((5, 2.5, 1.0, 12345), (2, 2.76, 0.9, 1), (8, 3.5, 0.9, 99),
+                           (10, 3.5, 0.9, 2**40 + 3)):
+        ns = 3000
+        eng = ce.BatchedRK45(ce.Molecule([ce.Atom("X", 1000.0, 1.0)] * n), 1.0, ns, 1e-6, 1e-9)
+        eng.sample(seed=seed, n_sims=ns, radius=R, min_separation=ms, first_index=777)
+        got = eng.geometries()
+        ref, st, fails = oracle.device_sample(seed, 777, ns, n, R, ms, 10000)
+        assert fails == 0
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_ch4_headline_rule_vs_oracle_lane_f32(gpu, oracle):
+    """The headline configuration: fixed h0 1e-3, rtol 1e-6, atol 1e-9, stop on
+    momentum change < 1e-7 or t = 1e4, f32 state."""
+    ce = gpu
+    _, geom = oracle.sample(12345, 5, 2000, 2.5, 1.0)
+    eng, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **BASELINE_RULE)
+    from oracle import ffi
+    ref = oracle.lane_run(CH4_MASSES, CH4_CHARGES, geom,
+                          ffi.make_config(rtol=1e-6, atol=1e-9, **BASELINE_RULE))
+    err = per_atom_rel(res.momenta, ref.momenta)
+    assert err.max() < 1e-5, err.max()
+    assert np.mean(res.steps == ref.steps) > 0.98
+    np.testing.assert_array_equal(res.status, ref.status)
+    np.testing.assert_allclose(res.t_final, ref.t_final, rtol=0, atol=0)
+    st = eng.stats()
+    assert st["accepted_steps"] == int(res.steps.sum())
+    assert st["attempted_steps"] == int(res.attempts.sum())
+
+
+def test_ch4_f32_vs_reference_forms(gpu, oracle):
+    """Against the reference's own engines: the f32 scalar build
+    (COULOMB_SINGLE_PRECISION) and the lockstep BatchedRK45<float> restatement."""
+    ce = gpu
+    from oracle import ffi
+    _, geom = oracle.sample(49188291, 5, 512, 2.5, 1.0)
+    for rule in (BASELINE_RULE, dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7)):
+        eng, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **rule)
+        cfg = ffi.make_config(rtol=1e-6, atol=1e-9, **rule)
+        sc = oracle.scalar_run(CH4_MASSES, CH4_CHARGES, geom, cfg, precision="f32")
+        bt = oracle.batched_run(CH4_MASSES, CH4_CHARGES, geom, cfg, lanes=16)
+        assert per_atom_rel(res.momenta, sc.momenta).max() < 1e-5
+        assert per_atom_rel(res.momenta, bt.momenta).max() < 1e-5
+
+
+def test_f64_engine_matches_oracle_lane_f64(gpu, oracle):
+    ce = gpu
+    from oracle import ffi
+    _, geom = oracle.sample(777, 5, 300, 2.5, 1.0)
+    for rule in (dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3),
+                 dict(stop_rule=0, pe_stop_fraction=1e-6)):
+        _, res = engine_run(ce, ch4(ce), geom, 1e-10, 1e-14, precision=1, **rule)
+        ref = oracle.lane_run(CH4_MASSES, CH4_CHARGES, geom,
+                              ffi.make_config(rtol=1e-10, atol=1e-14, **rule), precision="f64")
+        np.testing.assert_array_equal(res.steps, ref.steps)
+        assert per_atom_rel(res.momenta, ref.momenta).max() < 1e-12
+
+
+def test_pe_rule_reference_defaults_vs_ref_engine(gpu, oracle):
+    """The reference's own semantics (pe_stop, Hairer start, redistribution) on the
+    golden fixture molecules and CH4, against the reference's f64 engine."""
+    ce = gpu
+    from oracle import ffi
+    _, geom = oracle.sample(4242, 5, 128, 4.0, 0.25)
+    cfg = dict(stop_rule=0, pe_stop_fraction=1e-5)
+    _, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **cfg)
+    ref = oracle.scalar_run(CH4_MASSES, CH4_CHARGES, geom,
+                            ffi.make_config(rtol=1e-6, atol=1e-9, **cfg), precision="f64")
+    assert (res.status == ce.SIM_OK_PE).all()
+    assert per_atom_rel(res.momenta, ref.momenta).max() < 1e-5
