@@ -129,7 +129,25 @@ def main() -> int:
     ap.add_argument("--refill-threshold", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune the launch shape on this workload first (cached)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (HBM counter "
+                         "calibration for rocprofv3 --pmc) and exit")
     args = ap.parse_args()
+
+    if args.copy_probe_mb:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
+                          "launches": 3}))
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py: this engine is measured on one GPU; --gpus {args.gpus} is not "
@@ -144,9 +162,26 @@ def main() -> int:
         return 1
 
     eng = ce.BatchedRK45(ch4_molecule(ce), 1.0, args.sims, RTOL, ATOL)
-    eng.set_launch(refill_threshold=args.refill_threshold, chunk=args.chunk,
-                   blocks_per_cu=args.blocks_per_cu)
     cfg = run_config(ce)
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "coulomb_explosion_amd.autotune", os.path.join(os.path.dirname(ce.__file__), "autotune.py"))
+    autotune = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(autotune)
+    tuned = None
+    if args.autotune:
+        eng.sample(SEED, args.sims, RADIUS, MIN_SEP)
+        entry = autotune.tune(ce, eng, cfg, log=lambda m: print(m, file=sys.stderr, flush=True))
+        tuned = entry["shape"]
+    else:
+        tuned = autotune.apply_cached(ce, eng, cfg)
+    if any((args.refill_threshold, args.chunk, args.blocks_per_cu, args.block_threads)):
+        shape = dict(tuned or {})
+        for k, v in (("refill_threshold", args.refill_threshold), ("chunk", args.chunk),
+                     ("blocks_per_cu", args.blocks_per_cu), ("block_threads", args.block_threads)):
+            if v:
+                shape[k] = v
+        eng.set_launch(**shape)
 
     def one_step(seed_offset: int):
         eng.sample(SEED + seed_offset, args.sims, RADIUS, MIN_SEP)

@@ -70,7 +70,8 @@ class _Results(C.Structure):
 
 
 class _RunStats(C.Structure):
-    _fields_ = [("kernel_ms", C.c_double), ("sampler_ms", C.c_double), ("sims", C.c_uint64),
+    _fields_ = [("kernel_ms", C.c_double), ("sampler_ms", C.c_double), ("pack_ms", C.c_double),
+                ("sims", C.c_uint64),
                 ("attempted_steps", C.c_uint64), ("accepted_steps", C.c_uint64),
                 ("refills", C.c_uint64), ("grid_blocks", C.c_int32),
                 ("block_threads", C.c_int32), ("n_cus", C.c_int32), ("pad_", C.c_int32)]
@@ -104,6 +105,8 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_engine_set_launch": ([P, P], C.c_int),
         "ce_engine_get_launch": ([P, P], C.c_int),
         "ce_engine_enable_trace": ([P, C.c_uint32], C.c_int),
+        "ce_record_bytes": ([C.c_int32], C.c_size_t),
+        "ce_engine_pack_records": ([P, C.c_uint64, S, P], C.c_int),
         "ce_host_sampler_create": ([C.c_uint64, C.c_double, C.c_double, C.c_uint64, P], C.c_int),
         "ce_host_sampler_reseed": ([P, C.c_uint64], C.c_int),
         "ce_host_sampler_sample": ([P, C.c_int32, S, P], C.c_int),
@@ -366,10 +369,25 @@ class BatchedRK45:
         return Result(mom.reshape(-1), steps, conv, int(steps.max()) if ns else 0, st, att, tf,
                       ee, mom32, tr)
 
+    def records(self, first: int = 0, n_sims: int | None = None) -> np.ndarray:
+        """Packed dataset records (examples/generate_dataset.cpp:198-217) as a
+        numpy structured array with the dtype of examples/to_parquet.py:34-42."""
+        n_sims = self._loaded - first if n_sims is None else n_sims
+        rb = _lib.ce_record_bytes(self.n)
+        buf = np.zeros(n_sims * rb, dtype=np.uint8)
+        _check(_lib.ce_engine_pack_records(self._h, first, n_sims, buf.ctypes.data))
+        return buf.view(record_dtype(self.n))
+
     def stats(self) -> dict:
         s = _RunStats()
         _check(_lib.ce_engine_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in _RunStats._fields_ if f != "pad_"}
+
+
+def record_dtype(n_atoms: int) -> np.dtype:
+    """The record layout as numpy reads it (examples/to_parquet.py:34-42)."""
+    return np.dtype([("init_pos", "<f4", (n_atoms * 3,)), ("final_p", "<f4", (n_atoms * 3,)),
+                     ("steps", "<u4"), ("converged", "u1")])
 
 
 def batched_accelerations(n: int, k: int, px, py, pz, pair_const, inv_mass):

@@ -66,6 +66,9 @@ struct ce_engine {
   double* eerr = nullptr;
   double* trace = nullptr;
   uint32_t trace_cap = 0;
+  uint8_t* records = nullptr;   // device staging of packed records (lazy)
+  size_t records_cap = 0;       // sims it holds
+  hipEvent_t p0 = nullptr, p1 = nullptr;
   // launch
   ce_launch_config launch = {0, 0, 0, 0};
   int n_cus = 0;
@@ -176,14 +179,16 @@ int copy_results(ce_engine* e, ce_results* out) {
 void free_engine(ce_engine* e) {
   if (e == nullptr) return;
   hipSetDevice(e->device);
-  void* bufs[] = {e->geom, e->gstat,    e->ctr,    e->mom,  e->mom32, e->steps,
-                  e->attempts, e->status, e->tfin, e->eerr, e->trace};
+  void* bufs[] = {e->geom,     e->gstat,  e->ctr,  e->mom,  e->mom32,  e->steps,
+                  e->attempts, e->status, e->tfin, e->eerr, e->trace, e->records};
   for (void* b : bufs)
     if (b) hipFree(b);
   if (e->k0) hipEventDestroy(e->k0);
   if (e->k1) hipEventDestroy(e->k1);
   if (e->s0) hipEventDestroy(e->s0);
   if (e->s1) hipEventDestroy(e->s1);
+  if (e->p0) hipEventDestroy(e->p0);
+  if (e->p1) hipEventDestroy(e->p1);
   if (e->stream) hipStreamDestroy(e->stream);
   delete e;
 }
@@ -258,11 +263,14 @@ int ce_engine_create(const ce_molecule_desc* mol, const ce_tolerances* tol, size
   };
   chk(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties");
   if (rc == CE_OK) e->n_cus = prop.multiProcessorCount;
+  e->stats.n_cus = e->n_cus;
   chk(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
   chk(hipEventCreate(&e->k0), "hipEventCreate");
   chk(hipEventCreate(&e->k1), "hipEventCreate");
   chk(hipEventCreate(&e->s0), "hipEventCreate");
   chk(hipEventCreate(&e->s1), "hipEventCreate");
+  chk(hipEventCreate(&e->p0), "hipEventCreate");
+  chk(hipEventCreate(&e->p1), "hipEventCreate");
   const size_t n3 = 3 * static_cast<size_t>(e->n);
   if (rc == CE_OK) rc = hip_alloc(&e->geom, capacity * n3);
   if (rc == CE_OK) rc = hip_alloc(&e->gstat, capacity);
@@ -473,6 +481,51 @@ int ce_engine_download(ce_engine* e, ce_results* out) {
 int ce_engine_stats(const ce_engine* e, ce_run_stats* out) {
   if (e == nullptr || out == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   *out = e->stats;
+  return CE_OK;
+}
+
+size_t ce_record_bytes(int32_t n_atoms) {
+  return n_atoms > 0 ? static_cast<size_t>(24 * n_atoms + 5) : 0;
+}
+
+int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t* host_out) {
+  if (e == nullptr || (host_out == nullptr && n_sims > 0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!e->have_results) return fail(CE_ERR_RUNTIME, "no results: run the engine first");
+  if (first > e->n_loaded || n_sims > e->n_loaded - first)
+    return fail(CE_ERR_INVALID_ARGUMENT, "record range exceeds the loaded sims");
+  if (n_sims == 0) return CE_OK;
+  CE_HIP(hipSetDevice(e->device));
+  const size_t rb = ce_record_bytes(e->n);
+  if (e->records == nullptr || e->records_cap < n_sims) {
+    if (e->records) CE_HIP(hipFree(e->records));
+    e->records = nullptr;
+    e->records_cap = 0;
+    const int rc = hip_alloc(&e->records, n_sims * rb);
+    if (rc != CE_OK) return rc;
+    e->records_cap = n_sims;
+  }
+  const dim3 grid(static_cast<unsigned>((n_sims + ce::kRecBlock - 1) / ce::kRecBlock));
+  const dim3 block(ce::kRecBlock);
+  CE_HIP(hipEventRecord(e->p0, e->stream));
+  switch (e->n) {
+#define CE_PACK(NN)                                                                            \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL(ce::pack_records<NN>, grid, block, 0, e->stream, e->geom, e->mom32,     \
+                       e->steps, e->status, first, static_cast<uint64_t>(n_sims), e->records); \
+    break;
+    CE_PACK(1) CE_PACK(2) CE_PACK(3) CE_PACK(4) CE_PACK(5) CE_PACK(6) CE_PACK(7) CE_PACK(8)
+    CE_PACK(9) CE_PACK(10)
+#undef CE_PACK
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CE_HIP(hipGetLastError());
+  CE_HIP(hipEventRecord(e->p1, e->stream));
+  CE_HIP(hipMemcpyAsync(host_out, e->records, n_sims * rb, hipMemcpyDeviceToHost, e->stream));
+  CE_HIP(hipStreamSynchronize(e->stream));
+  float ms = 0.0f;
+  CE_HIP(hipEventElapsedTime(&ms, e->p0, e->p1));
+  e->stats.pack_ms = ms;
   return CE_OK;
 }
 

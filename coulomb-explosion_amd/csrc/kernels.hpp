@@ -660,6 +660,51 @@ __global__ void __launch_bounds__(256) soa_pe(uint64_t k, const float* px, const
   pe[l] = u;
 }
 
+// ------------------------------------------------------------ records ----
+// Dataset records of examples/generate_dataset.cpp:198-217, packed little-endian
+// with no padding: n*3 f32 initial positions, n*3 f32 final momenta, u32 steps,
+// u8 converged -- 24n + 5 bytes (125 for CH4).  A block packs kRecBlock
+// consecutive sims into LDS byte by byte, then streams the contiguous span out
+// with 16-byte stores (the span starts at a multiple of 16: kRecBlock * bytes is
+// a multiple of 16 because kRecBlock is).
+constexpr int kRecBlock = 256;
+
+__device__ __forceinline__ void put_bytes(uint8_t* dst, uint32_t v) {
+  dst[0] = static_cast<uint8_t>(v);
+  dst[1] = static_cast<uint8_t>(v >> 8);
+  dst[2] = static_cast<uint8_t>(v >> 16);
+  dst[3] = static_cast<uint8_t>(v >> 24);
+}
+
+template <int N>
+__global__ void __launch_bounds__(kRecBlock) pack_records(const double* geom, const float* mom32,
+                                                          const uint32_t* steps,
+                                                          const uint8_t* status, uint64_t first,
+                                                          uint64_t n_sims, uint8_t* out) {
+  constexpr int kBytes = 24 * N + 5;
+  __shared__ __align__(16) uint8_t stage[kRecBlock * kBytes];
+  const uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * kRecBlock;  // relative to first
+  const uint64_t s = b0 + threadIdx.x;
+  if (s < n_sims) {
+    const uint64_t g = first + s;
+    uint8_t* r = stage + threadIdx.x * kBytes;
+#pragma unroll
+    for (int q = 0; q < 3 * N; ++q) put_bytes(r + 4 * q, __float_as_uint(static_cast<float>(geom[g * 3 * N + q])));
+#pragma unroll
+    for (int q = 0; q < 3 * N; ++q) put_bytes(r + 12 * N + 4 * q, __float_as_uint(mom32[g * 3 * N + q]));
+    put_bytes(r + 24 * N, steps[g]);
+    r[24 * N + 4] = status[g] <= 2 ? 1 : 0;
+  }
+  __syncthreads();
+  const uint64_t count = (n_sims - b0 < kRecBlock) ? (n_sims - b0) : kRecBlock;
+  const uint64_t bytes = count * kBytes;
+  uint8_t* dst = out + b0 * kBytes;
+  const uint64_t n16 = bytes / 16;
+  for (uint64_t i = threadIdx.x; i < n16; i += kRecBlock)
+    reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(stage)[i];
+  for (uint64_t i = n16 * 16 + threadIdx.x; i < bytes; i += kRecBlock) dst[i] = stage[i];
+}
+
 __global__ void zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;

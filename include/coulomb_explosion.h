@@ -120,6 +120,7 @@ typedef struct {
 typedef struct {
   double kernel_ms;          /* integration kernel, HIP events on the engine stream */
   double sampler_ms;         /* last device-sampler kernel */
+  double pack_ms;            /* last record-pack kernel */
   uint64_t sims;
   uint64_t attempted_steps;  /* summed over sims */
   uint64_t accepted_steps;
@@ -178,6 +179,15 @@ int ce_engine_get_launch(const ce_engine* eng, ce_launch_config* lc);
 /* Records the first trace_cap accepted step sizes of every sim on the next
  * runs (the reference's StepObserver hook, driver.hpp:12-18); 0 disables. */
 int ce_engine_enable_trace(ce_engine* eng, uint32_t trace_cap);
+
+/* Dataset records of the reference generator (examples/generate_dataset.cpp:
+ * 198-217; numpy dtype examples/to_parquet.py:34-42): per sim, packed
+ * little-endian, no padding -- n*3 f32 initial positions, n*3 f32 final momenta
+ * (post-redistribution m*v), u32 accepted steps, u8 converged.  24n + 5 bytes. */
+size_t ce_record_bytes(int32_t n_atoms);
+/* Packs sims [first, first + n_sims) of the last run into records on the device
+ * and copies them to host_out (n_sims * ce_record_bytes(n) bytes). */
+int ce_engine_pack_records(ce_engine* eng, uint64_t first, size_t n_sims, uint8_t* host_out);
 
 /* Host sampler with the reference's exact stream: UniformSphereSampler(Options,
  * seed) (sampler.hpp:46-72, sampler.cpp:7-57) -- std::mt19937_64 feeding

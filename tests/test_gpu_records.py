@@ -1,0 +1,62 @@
+"""Dataset records (examples/generate_dataset.cpp:198-217) written by the device
+pack kernel, and the generator CLI; byte layout as examples/to_parquet.py:34-42
+reads it."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import CH4_MASSES, ROOT
+
+pytestmark = pytest.mark.gpu
+
+
+def test_records_match_results(gpu, oracle):
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] +
+                      [ce.Atom("H", m, 1.0) for m in CH4_MASSES[1:]])
+    ns = 1000  # not a multiple of the 256-record block: exercises the tail
+    eng = ce.BatchedRK45(mol, 1.0, ns, 1e-6, 1e-9)
+    eng.sample(seed=3, n_sims=ns, radius=2.5, min_separation=1.0)
+    res = eng.run(ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, h0_mode=ce.H0_FIXED))
+    rec = eng.records()
+    assert rec.dtype.itemsize == 125 and rec.shape == (ns,)
+    geom = eng.geometries()
+    np.testing.assert_array_equal(rec["init_pos"], geom.reshape(ns, -1).astype(np.float32))
+    np.testing.assert_array_equal(rec["final_p"], res.momenta_f32.reshape(ns, -1))
+    np.testing.assert_array_equal(rec["final_p"], res.momenta.reshape(ns, -1).astype(np.float32))
+    np.testing.assert_array_equal(rec["steps"], res.steps)
+    np.testing.assert_array_equal(rec["converged"], res.converged)
+    part = eng.records(first=300, n_sims=257)
+    np.testing.assert_array_equal(part, rec[300:557])
+    # raw bytes: little-endian packed, no padding
+    raw = rec.tobytes()
+    first = np.frombuffer(raw[:60], dtype="<f4")
+    np.testing.assert_array_equal(first, geom[0].reshape(-1).astype(np.float32))
+    assert raw[124] == rec["converged"][0]
+
+
+def test_generator_cli(gpu, tmp_path):
+    exe = os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset")
+    out = str(tmp_path / "ch4.bin")
+    r = subprocess.run([exe, "--out", out, "--sims", "5000", "--batch", "2048", "--radius", "2.5",
+                        "--min-separation", "1.0", "--rtol", "1e-6", "--atol", "1e-9",
+                        "--stop-rule", "tmax", "--h0", "1e-3", "--seed", "11"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    meta = json.load(open(out + ".meta.json"))
+    assert meta["n_sims"] == 5000 and meta["n_atoms"] == 5 and meta["n_failed_convergence"] == 0
+    assert meta["symbols"] == ["C", "H", "H", "H", "H"]
+    from __graft_entry__ import load_package
+    ce = load_package()
+    rec = np.fromfile(out, dtype=ce.record_dtype(5))
+    assert rec.shape == (5000,) and (rec["converged"] == 1).all()
+    # batches are keyed by global sim index: the same sims as one 5000-sim run
+    masses = [12.011 * 1822.888486209] + [1.008 * 1822.888486209] * 4
+    mol = ce.Molecule([ce.Atom("C", masses[0], 1.0)] + [ce.Atom("H", m, 1.0) for m in masses[1:]])
+    eng = ce.BatchedRK45(mol, 1.0, 5000, 1e-6, 1e-9)
+    eng.sample(seed=11, n_sims=5000, radius=2.5, min_separation=1.0)
+    eng.run(ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, h0_mode=ce.H0_FIXED, h0=1e-3))
+    np.testing.assert_array_equal(eng.records(), rec)
