@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
-"""Benchmark: converged CH4 Coulomb-explosion sims per second on one MI355X.
+"""Benchmark: converged Coulomb-explosion sims per second on one MI355X.
 
-Workload (one "step" = one pass of the hot path over one batch):
+Default workload ("ch4", the headline; one "step" = one pass of the hot path over
+one batch):
   10M CH4 sims (C 21874.66, 4 x H 1837.15 electron masses, all charges +1),
   positions uniform in a ball of radius 2.5 bohr, pairs >= 1.0 bohr apart
   (device counter-based sampler), atoms at rest; f32 state, DP5(4) rtol 1e-6,
@@ -10,15 +11,23 @@ Workload (one "step" = one pass of the hot path over one batch):
   written to HBM.  The timed region is sampler + integrator, inputs and
   outputs resident in HBM.
 
+Other workloads (--workload):
+  mixed  10M sims in one launch, 25% each N2 (2 x 25526.04), CO2 (C 21874.66,
+         2 x O 29156.95), CH4, C2H6 (2 C, 6 H); per-atom charges uniform in
+         {1, 2}; R = N^(1/3) + 1.5, min sep 0.9; same tolerances and stop rule.
+  n10    10M sims of a 10-atom molecule (2 C, 2 O, 6 H), charges uniform in
+         {1, 2, 3}, R 3.5, min sep 0.9 (the plain method on that distribution).
+
 Also reported: the error of the f32 results against the engine's FP64 path at
-rtol 1e-10 on a 100k-sim subset, the integrator kernel's achieved FP32 rate
-against the vector FP32 peak (HIP events on the engine stream), and the CPU
-baseline (the lockstep BatchedRK45<float> restatement, one instance per host
-thread) timed in the same run.
+rtol 1e-10 on a subset, the integrator kernel's achieved FP32 rate against the
+FP32 peak (HIP events on the engine stream), and (ch4) the CPU baseline (the
+lockstep BatchedRK45<float> restatement, one instance per host thread) timed in
+the same run.
 """
 from __future__ import annotations
 
 import argparse
+import importlib.util
 import json
 import os
 import sys
@@ -30,13 +39,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "converged sims/sec on 1 MI355X at ≤1e-5 momentum err: closure vs plain RK45"
-CH4_MASSES = [21874.66] + [1837.15] * 4
+C_, O_, H_, N14 = 21874.66, 29156.95, 1837.15, 25526.04
+CH4_MASSES = [C_] + [H_] * 4
 CH4_CHARGES = [1.0] * 5
 RTOL, ATOL, H0, T_MAX, MOM_TOL = 1e-6, 1e-9, 1e-3, 1e4, 1e-7
 RADIUS, MIN_SEP = 2.5, 1.0
 SEED = 49188291
-PEAK_FP32_TFLOPS = 157.3   # MI355X published vector/matrix FP32 peak
-PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
+MIXED = [([N14, N14], 2), ([C_, O_, O_], 3), (CH4_MASSES, 5), ([C_, C_] + [H_] * 6, 8)]
+N10_MASSES = [C_, C_, O_, O_] + [H_] * 6
 
 
 def flops_per_attempt(n: int) -> int:
@@ -54,41 +65,99 @@ def flops_per_sim_fixed(n: int) -> int:
     return (20 * p + 3 * n) + 10 * n
 
 
-def ch4_molecule(ce):
-    return ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] +
-                       [ce.Atom("H", m, 1.0) for m in CH4_MASSES[1:]])
-
-
 def run_config(ce, precision=0):
     return ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, t_max=T_MAX, momentum_tol=MOM_TOL,
                         h0_mode=ce.H0_FIXED, h0=H0, precision=precision)
 
 
-def accuracy_subset(ce, n_sub: int) -> dict:
-    """f32 engine vs the engine's FP64 path (rtol 1e-10, atol 1e-14) on the
-    first n_sub sims of the workload (same seed, same geometries)."""
-    mol = ch4_molecule(ce)
-    e32 = ce.BatchedRK45(mol, 1.0, n_sub, RTOL, ATOL)
-    e32.sample(SEED, n_sub, RADIUS, MIN_SEP)
-    r32 = e32.run(run_config(ce))
-    e64 = ce.BatchedRK45(mol, 1.0, n_sub, 1e-10, 1e-14)
-    e64.sample(SEED, n_sub, RADIUS, MIN_SEP)
-    r64 = e64.run(run_config(ce, precision=ce.PRECISION_F64))
-    p32 = r32.momenta.reshape(n_sub, 5, 3)
-    p64 = r64.momenta.reshape(n_sub, 5, 3)
-    per_atom = (np.linalg.norm(p32 - p64, axis=-1) / np.linalg.norm(p64, axis=-1)).max(axis=-1)
-    cfg_norm = (np.linalg.norm((p32 - p64).reshape(n_sub, -1), axis=1) /
-                np.linalg.norm(p64.reshape(n_sub, -1), axis=1))
+def load_autotune(ce):
+    spec = importlib.util.spec_from_file_location(
+        "coulomb_explosion_amd.autotune", os.path.join(os.path.dirname(ce.__file__), "autotune.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+class Workload:
+    """A batch of sims of one or more molecule sizes on one engine."""
+
+    def __init__(self, ce, kind: str, sims: int, rtol=RTOL, atol=ATOL):
+        self.ce, self.kind, self.sims = ce, kind, sims
+        if kind == "ch4":
+            mol = ce.Molecule([ce.Atom("C", C_, 1.0)] + [ce.Atom("H", H_, 1.0)] * 4)
+            self.eng = ce.BatchedRK45(mol, 1.0, sims, rtol, atol)
+            self.sizes = [(5, sims)]
+        else:
+            if kind == "mixed":
+                per = [sims // 4] * 4
+                per[-1] += sims - sum(per)
+                sp = [ce.Species(ce.Molecule([ce.Atom("X", m, 1.0) for m in masses]), cnt,
+                                 n ** (1 / 3) + 1.5, 0.9, charge_levels=2, max_total_charge=2 * n)
+                      for (masses, n), cnt in zip(MIXED, per)]
+            elif kind == "n10":
+                sp = [ce.Species(ce.Molecule([ce.Atom("X", m, 1.0) for m in N10_MASSES]), sims,
+                                 3.5, 0.9, charge_levels=3)]
+            else:
+                raise ValueError(kind)
+            self.eng = ce.MixedBatch(sp, 1.0, rtol, atol)
+            self.sizes = [(s.molecule.size(), s.count) for s in sp]
+
+    def sample(self, seed=SEED):
+        if self.kind == "ch4":
+            self.eng.sample(seed, self.sims, RADIUS, MIN_SEP)
+        else:
+            self.eng.sample(seed)
+
+    def run(self, cfg):
+        self.eng.run(cfg, download=False)
+
+    def momenta_status(self):
+        """[(momenta [count, n, 3], status, attempts)] per size."""
+        if self.kind == "ch4":
+            r = self.eng.download()
+            return [(r.momenta.reshape(-1, 5, 3), r.status, r.attempts)]
+        return [(r["momenta"], r["status"], r["attempts"]) for r in self.eng.download()]
+
+    def describe(self) -> str:
+        return {
+            "ch4": "CH4 plain method, 10M sims per step: R 2.5, min sep 1.0, rtol 1e-6, atol "
+                   "1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4 (FP64 t), f32 state",
+            "mixed": "mixed N=2/3/5/8 (N2, CO2, CH4, C2H6) 25% each in one persistent launch, "
+                     "per-atom charges uniform {1,2}, R = N^(1/3)+1.5, min sep 0.9, rtol 1e-6, "
+                     "atol 1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4, f32 state",
+            "n10": "N=10 (2C, 2O, 6H), charges uniform {1,2,3}, R 3.5, min sep 0.9, plain "
+                   "method, rtol 1e-6, atol 1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4",
+        }[self.kind]
+
+
+def accuracy_subset(ce, kind: str, n_sub: int) -> dict:
+    """f32 engine vs the engine's FP64 path (rtol 1e-10, atol 1e-14) on n_sub
+    sims of the workload's distribution (same seed, same geometries/charges)."""
+    w32 = Workload(ce, kind, n_sub)
+    w64 = Workload(ce, kind, n_sub, rtol=1e-10, atol=1e-14)
+    for w, prec in ((w32, 0), (w64, 1)):
+        w.sample()
+        w.run(run_config(ce, prec))
+    per_atom, cfg_norm, steps64 = [], [], []
+    for (p32, _, _), (p64, _, a64) in zip(w32.momenta_status(), w64.momenta_status()):
+        num = np.linalg.norm(p32 - p64, axis=-1)
+        per_atom.append((num / np.linalg.norm(p64, axis=-1)).max(axis=-1))
+        k = p64.shape[0]
+        cfg_norm.append(np.linalg.norm((p32 - p64).reshape(k, -1), axis=1) /
+                        np.linalg.norm(p64.reshape(k, -1), axis=1))
+        steps64.append(a64)
+    per_atom = np.concatenate(per_atom)
+    cfg_norm = np.concatenate(cfg_norm)
     return {
         "subset": n_sub,
-        "vs": "engine FP64 path, rtol 1e-10 atol 1e-14, same stop rule",
+        "vs": "engine FP64 path, rtol 1e-10 atol 1e-14, same stop rule and geometries",
         "per_atom_rel_max": float(per_atom.max()),
         "per_atom_rel_p99": float(np.percentile(per_atom, 99)),
         "per_atom_rel_median": float(np.median(per_atom)),
         "config_norm_rel_max": float(cfg_norm.max()),
         "within_1e-5": float(np.mean(per_atom <= 1e-5)),
-        "fp64_mean_steps": float(r64.steps.mean()),
-        "fp64_kernel_ms": e64.stats()["kernel_ms"],
+        "fp64_mean_attempts": float(np.concatenate(steps64).mean()),
+        "fp64_kernel_ms": w64.eng.stats()["kernel_ms"],
     }
 
 
@@ -120,6 +189,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["ch4", "mixed", "n10"], default="ch4")
     ap.add_argument("--sims", type=int, default=10_000_000)
     ap.add_argument("--accuracy-subset", type=int, default=100_000)
     ap.add_argument("--cpu-sims", type=int, default=400_000)
@@ -161,70 +231,70 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
 
-    eng = ce.BatchedRK45(ch4_molecule(ce), 1.0, args.sims, RTOL, ATOL)
+    w = Workload(ce, args.workload, args.sims)
     cfg = run_config(ce)
-    import importlib.util
-    spec = importlib.util.spec_from_file_location(
-        "coulomb_explosion_amd.autotune", os.path.join(os.path.dirname(ce.__file__), "autotune.py"))
-    autotune = importlib.util.module_from_spec(spec)
-    spec.loader.exec_module(autotune)
-    tuned = None
+    autotune = load_autotune(ce)
     if args.autotune:
-        eng.sample(SEED, args.sims, RADIUS, MIN_SEP)
-        entry = autotune.tune(ce, eng, cfg, log=lambda m: print(m, file=sys.stderr, flush=True))
+        w.sample()
+        entry = autotune.tune(ce, w.eng, cfg, log=lambda m: print(m, file=sys.stderr, flush=True),
+                              key_extra=args.workload)
         tuned = entry["shape"]
     else:
-        tuned = autotune.apply_cached(ce, eng, cfg)
+        tuned = autotune.apply_cached(ce, w.eng, cfg, key_extra=args.workload)
     if any((args.refill_threshold, args.chunk, args.blocks_per_cu, args.block_threads)):
         shape = dict(tuned or {})
         for k, v in (("refill_threshold", args.refill_threshold), ("chunk", args.chunk),
                      ("blocks_per_cu", args.blocks_per_cu), ("block_threads", args.block_threads)):
             if v:
                 shape[k] = v
-        eng.set_launch(**shape)
+        w.eng.set_launch(**shape)
 
-    def one_step(seed_offset: int):
-        eng.sample(SEED + seed_offset, args.sims, RADIUS, MIN_SEP)
-        s = eng.stats()
-        eng.run(cfg, download=False)
-        return s["sampler_ms"], eng.stats()
-
-    print(f"[bench] warmup {args.warmup} x {args.sims} CH4 sims", file=sys.stderr, flush=True)
-    for w in range(args.warmup):
-        one_step(0)
-    kernel_ms, sampler_ms, attempts, accepted = [], [], 0, 0
+    print(f"[bench] {args.workload}: warmup {args.warmup} x {args.sims} sims", file=sys.stderr,
+          flush=True)
+    for _ in range(args.warmup):
+        w.sample()
+        w.run(cfg)
+    kernel_ms, sampler_ms, attempts, accepted, launches = [], [], 0, 0, 0
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    for k in range(args.steps):
-        sms, st = one_step(0)
-        sampler_ms.append(sms)
+    for _ in range(args.steps):
+        w.sample()
+        sampler_ms.append(w.eng.stats()["sampler_ms"])
+        w.run(cfg)
+        st = w.eng.stats()
         kernel_ms.append(st["kernel_ms"])
         attempts += st["attempted_steps"]
         accepted += st["accepted_steps"]
+        launches = st["launches"]
     elapsed = time.perf_counter() - t0
-    res = eng.download()
-    n_ok = int(np.sum(res.status <= ce.SIM_OK_TMAX))
-    stats = eng.stats()
+    per_size = w.momenta_status()
+    n_ok = int(sum(np.sum(s <= ce.SIM_OK_TMAX) for _, s, _ in per_size))
+    stats = w.eng.stats()
 
     value = n_ok * args.steps / elapsed
     avg_kernel_s = float(np.mean(kernel_ms)) / 1e3
-    att_per_launch = attempts / args.steps
-    flops_launch = att_per_launch * flops_per_attempt(5) + args.sims * flops_per_sim_fixed(5)
+    # algorithmic FLOPs of the last launch from its per-sim attempt counts
+    flops_launch = sum(float(a.astype(np.float64).sum()) * flops_per_attempt(n) +
+                       cnt * flops_per_sim_fixed(n)
+                       for (n, cnt), (_, _, a) in zip(w.sizes, per_size))
     achieved_tflops = flops_launch / avg_kernel_s / 1e12
-    # HBM bytes the launch must move: geometry in (120 B/sim, f64), results out
-    # (momenta f64 120 + f32 60, steps 4, attempts 4, status 1, t 8, energy 8).
-    bytes_launch = args.sims * (120 + 120 + 60 + 4 + 4 + 1 + 8 + 8)
+    bytes_launch = sum(cnt * (24 * n + 24 * n + 12 * n + 25) for n, cnt in w.sizes)
 
-    print(f"[bench] accuracy subset {args.accuracy_subset}", file=sys.stderr, flush=True)
-    acc = accuracy_subset(ce, args.accuracy_subset) if args.accuracy_subset > 0 else None
+    acc = None
+    if args.accuracy_subset > 0:
+        print(f"[bench] accuracy subset {args.accuracy_subset}", file=sys.stderr, flush=True)
+        acc = accuracy_subset(ce, args.workload, args.accuracy_subset)
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "ch4":
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         print(f"[bench] cpu baseline {args.cpu_sims} sims x {args.cpu_reps} on {threads} threads",
               file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_sims, args.cpu_reps, threads)
 
+    kernel_name = ("dp5_persistent<float,5>" if args.workload == "ch4" else
+                   "dp5_mixed<float,2,3,5,8,per-sim charges>" if args.workload == "mixed" else
+                   "dp5_persistent<float,10,per-sim charges>")
     line = {
         "metric": METRIC,
         "value": value,
@@ -239,8 +309,7 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (device counter-based uniform-sphere sampler, seed %d)" % SEED,
         "config": {
-            "workload": "CH4 plain method, 10M sims per step: R 2.5, min sep 1.0, rtol 1e-6, "
-                        "atol 1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4 (FP64 t), f32 state",
+            "workload": w.describe(),
             "sims_per_step": args.sims,
             "method": "plain DP5(4), per-lane dt, device work-queue refill",
             "converged_per_step": n_ok,
@@ -248,12 +317,13 @@ def main() -> int:
             "mean_attempted_steps": attempts / (args.sims * args.steps),
             "kernel_ms_avg": avg_kernel_s * 1e3,
             "sampler_ms_avg": float(np.mean(sampler_ms)),
+            "launches_per_step": launches,
             "launch": {"grid": stats["grid_blocks"], "block": stats["block_threads"],
-                       "cus": stats["n_cus"], **eng.launch()},
+                       "cus": stats["n_cus"], **(tuned or {})},
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "dp5_persistent<float,5>",
+            "kernel": kernel_name,
             "achieved": achieved_tflops,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
@@ -261,11 +331,13 @@ def main() -> int:
             "traffic": None,
             "algorithmic_bytes_per_launch": bytes_launch,
             "hbm_gbs_algorithmic": bytes_launch / avg_kernel_s / 1e9,
-            "flops_per_attempt": flops_per_attempt(5),
+            "flops_per_attempt": {str(n): flops_per_attempt(n) for n, _ in w.sizes},
         },
         "accuracy": acc,
         "cpu_baseline": cpu,
     }
+    if cpu:
+        line["gpu_over_cpu"] = value / cpu["value"]
     print(json.dumps(line), flush=True)
     return 0
 

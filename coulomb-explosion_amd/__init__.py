@@ -74,7 +74,14 @@ class _RunStats(C.Structure):
                 ("sims", C.c_uint64),
                 ("attempted_steps", C.c_uint64), ("accepted_steps", C.c_uint64),
                 ("refills", C.c_uint64), ("grid_blocks", C.c_int32),
-                ("block_threads", C.c_int32), ("n_cus", C.c_int32), ("pad_", C.c_int32)]
+                ("block_threads", C.c_int32), ("n_cus", C.c_int32), ("launches", C.c_int32)]
+
+
+class _SpeciesDesc(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("charge_levels", C.c_int32),
+                ("masses", C.POINTER(C.c_double)), ("charges", C.POINTER(C.c_double)),
+                ("radius", C.c_double), ("min_separation", C.c_double), ("count", C.c_uint64),
+                ("max_total_charge", C.c_int32), ("pad_", C.c_int32)]
 
 
 class _LaunchConfig(C.Structure):
@@ -105,6 +112,10 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_engine_set_launch": ([P, P], C.c_int),
         "ce_engine_get_launch": ([P, P], C.c_int),
         "ce_engine_enable_trace": ([P, C.c_uint32], C.c_int),
+        "ce_engine_create_mixed": ([P, C.c_int32, C.c_double, P, P], C.c_int),
+        "ce_engine_sample_species": ([P, C.c_uint64, C.c_uint64, C.c_uint32], C.c_int),
+        "ce_engine_set_charges": ([P, P, S], C.c_int),
+        "ce_engine_get_charges": ([P, P, S], C.c_int),
         "ce_record_bytes": ([C.c_int32], C.c_size_t),
         "ce_engine_pack_records": ([P, C.c_uint64, S, P], C.c_int),
         "ce_host_sampler_create": ([C.c_uint64, C.c_double, C.c_double, C.c_uint64, P], C.c_int),
@@ -382,6 +393,141 @@ class BatchedRK45:
         s = _RunStats()
         _check(_lib.ce_engine_stats(self._h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in _RunStats._fields_ if f != "pad_"}
+
+    def set_charges(self, charges) -> None:
+        """Per-sim charges [n_sims, n] for the loaded geometries (replaces the
+        molecule's fixed charges on the next runs)."""
+        if self._dirty:
+            _check(_lib.ce_engine_set_geometries(self._h, self._geom.ctypes.data, self._loaded))
+            self._dirty = False
+        q = np.ascontiguousarray(charges, dtype=np.float64).reshape(self._loaded, self.n)
+        _check(_lib.ce_engine_set_charges(self._h, q.ctypes.data, self._loaded))
+
+
+@dataclass
+class Species:
+    """One molecule type of a mixed-size batch: `count` sims drawn in a ball of
+    `radius` with pairs >= `min_separation`; charge_levels > 0 gives every sim
+    its own charges uniform in {1..charge_levels} (sum <= max_total_charge if > 0)."""
+    molecule: Molecule
+    count: int
+    radius: float
+    min_separation: float
+    charge_levels: int = 0
+    max_total_charge: int = 0
+
+
+class MixedBatch:
+    """Several molecule sizes in one launch (no reference counterpart: the
+    reference batches one fixed molecule).  Results come back per species."""
+
+    def __init__(self, species: list, coulomb_k: float, rtol: float, atol: float):
+        self.species = list(species)
+        self._keep = []
+        descs = (_SpeciesDesc * len(self.species))()
+        for d, s in zip(descs, self.species):
+            m = np.ascontiguousarray(s.molecule.masses)
+            q = np.ascontiguousarray(s.molecule.charges)
+            self._keep += [m, q]
+            d.n_atoms = s.molecule.size()
+            d.charge_levels = s.charge_levels
+            d.masses = m.ctypes.data_as(C.POINTER(C.c_double))
+            d.charges = q.ctypes.data_as(C.POINTER(C.c_double))
+            d.radius = s.radius
+            d.min_separation = s.min_separation
+            d.count = s.count
+            d.max_total_charge = s.max_total_charge
+        tol = _Tolerances(float(rtol), float(atol))
+        h = C.c_void_p()
+        _check(_lib.ce_engine_create_mixed(descs, len(self.species), float(coulomb_k),
+                                           C.byref(tol), C.byref(h)))
+        self._h = h
+        self.total = sum(s.count for s in self.species)
+        self._per_sim_charges = any(s.charge_levels > 0 for s in self.species)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.ce_engine_destroy(h)
+            self._h = None
+
+    def _split(self, flat: np.ndarray, width: int) -> list:
+        out, o = [], 0
+        for s in self.species:
+            k = s.count * width * s.molecule.size()
+            shape = (s.count, s.molecule.size(), 3) if width == 3 else (s.count, s.molecule.size())
+            out.append(flat[o:o + k].reshape(shape))
+            o += k
+        return out
+
+    def _split_sims(self, a: np.ndarray) -> list:
+        out, o = [], 0
+        for s in self.species:
+            out.append(a[o:o + s.count])
+            o += s.count
+        return out
+
+    def sample(self, seed: int, first_index: int = 0, max_attempts: int = 10000) -> None:
+        _check(_lib.ce_engine_sample_species(self._h, seed, first_index, max_attempts))
+
+    def set_geometries(self, per_species: list) -> None:
+        flat = np.concatenate([np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
+                               for g in per_species])
+        _check(_lib.ce_engine_set_geometries(self._h, flat.ctypes.data, self.total))
+
+    def set_charges(self, per_species: list) -> None:
+        flat = np.concatenate([np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+                               for q in per_species])
+        _check(_lib.ce_engine_set_charges(self._h, flat.ctypes.data, self.total))
+
+    def geometries(self) -> list:
+        n = sum(s.count * 3 * s.molecule.size() for s in self.species)
+        flat = np.zeros(n)
+        _check(_lib.ce_engine_get_geometries(self._h, flat.ctypes.data, self.total))
+        return self._split(flat, 3)
+
+    def charges(self) -> list:
+        n = sum(s.count * s.molecule.size() for s in self.species)
+        flat = np.zeros(n)
+        _check(_lib.ce_engine_get_charges(self._h, flat.ctypes.data, self.total))
+        return self._split(flat, 1)
+
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0) -> None:
+        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk)
+        _check(_lib.ce_engine_set_launch(self._h, C.byref(lc)))
+
+    def stats(self) -> dict:
+        s = _RunStats()
+        _check(_lib.ce_engine_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _RunStats._fields_ if f != "pad_"}
+
+    def run(self, cfg: RunConfig | None = None, download: bool = True):
+        c = (cfg or RunConfig())._c()
+        _check(_lib.ce_engine_run(self._h, C.byref(c), None))
+        return self.download() if download else None
+
+    def download(self) -> list:
+        """Per species: dict(momenta [count, n, 3], steps, attempts, status, t_final, energy_err)."""
+        n = sum(s.count * 3 * s.molecule.size() for s in self.species)
+        mom = np.zeros(n)
+        steps = np.zeros(self.total, np.uint32)
+        att = np.zeros(self.total, np.uint32)
+        st = np.zeros(self.total, np.uint8)
+        tf = np.zeros(self.total)
+        ee = np.zeros(self.total)
+
+        def ptr(a, ct):
+            return a.ctypes.data_as(C.POINTER(ct))
+
+        res = _Results(ptr(mom, C.c_double), None, ptr(steps, C.c_uint32), ptr(att, C.c_uint32),
+                       ptr(st, C.c_uint8), ptr(tf, C.c_double), ptr(ee, C.c_double), None, 0, 0)
+        _check(_lib.ce_engine_download(self._h, C.byref(res)))
+        out = []
+        for i, m in enumerate(self._split(mom, 3)):
+            sl = [self._split_sims(a)[i] for a in (steps, att, st, tf, ee)]
+            out.append({"momenta": m, "steps": sl[0], "attempts": sl[1], "status": sl[2],
+                        "t_final": sl[3], "energy_err": sl[4]})
+        return out
 
 
 def record_dtype(n_atoms: int) -> np.dtype:

@@ -1,14 +1,14 @@
-"""Launch-shape autotuner for the persistent DP5(4) kernel.
+"""Launch-shape autotuner for the persistent DP5(4) kernels.
 
 The kernel's free parameters are the simulations per block (block_threads: one
 simulation per lane), the resident blocks per CU (occupancy actually launched),
 the refill threshold (idle lanes a wave waits for before it pulls new
 simulations) and the queue chunk (simulations a wave claims per atomic).  The
 pair loop and the stage sums are fully unrolled per molecule size at compile
-time, so "unroll" is fixed by N.  The tuner times the integration kernel (HIP
-events on the engine stream) on a sample of the real workload, coordinate by
-coordinate, and caches the best shape per (device, molecule size, precision,
-stop rule) in tuned_launch.json next to this file.
+time, so the unroll is fixed by N.  The tuner times the integration kernel (HIP
+events on the engine stream) on the engine's loaded workload, coordinate by
+coordinate, and caches the best shape per (device, workload, precision, stop
+rule) in tuned_launch.json next to this file.
 """
 from __future__ import annotations
 
@@ -20,16 +20,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CACHE = os.path.join(HERE, "tuned_launch.json")
 
 SPACE = {
-    "refill_threshold": [1, 2, 4, 8, 16, 32],
+    "refill_threshold": [1, 2, 4, 8, 16],
     "chunk": [64, 256, 1024],
     "blocks_per_cu": [0, 1, 2, 3, 4],
     "block_threads": [64, 128, 256],
 }
+DEFAULT = {"block_threads": 256, "blocks_per_cu": 0, "refill_threshold": 4, "chunk": 64}
 
 
-def _key(ce, eng, cfg) -> str:
+def _key(eng, cfg, key_extra: str) -> str:
     st = eng.stats()
-    return f"cus{st['n_cus']}-n{eng.atoms()}-p{cfg.precision}-s{cfg.stop_rule}"
+    return f"cus{st['n_cus']}-{key_extra}-p{cfg.precision}-s{cfg.stop_rule}"
 
 
 def load(key: str) -> dict | None:
@@ -58,10 +59,10 @@ def time_shape(eng, cfg, shape: dict, reps: int = 2) -> float:
     return best
 
 
-def tune(ce, eng, cfg, rounds: int = 1, log=print) -> dict:
+def tune(ce, eng, cfg, key_extra: str = "", rounds: int = 1, log=print) -> dict:
     """Coordinate descent over SPACE on the engine's loaded sims.  Returns
     {"shape": {...}, "kernel_ms": t, "trials": [...]} and caches it."""
-    shape = {"block_threads": 256, "blocks_per_cu": 0, "refill_threshold": 4, "chunk": 64}
+    shape = dict(DEFAULT)
     t0 = time.time()
     eng.run(cfg, download=False)  # warm
     best_ms = time_shape(eng, cfg, shape)
@@ -82,15 +83,15 @@ def tune(ce, eng, cfg, rounds: int = 1, log=print) -> dict:
                 if ms < best_ms:
                     best_ms, shape = ms, cand
     eng.set_launch(**shape)
-    entry = {"shape": shape, "kernel_ms": best_ms, "sims": eng.stats()["sims"],
-             "seconds": time.time() - t0, "trials": trials}
-    store(_key(ce, eng, cfg), entry)
+    entry = {"shape": shape, "kernel_ms": best_ms, "seconds": time.time() - t0,
+             "trials": trials}
+    store(_key(eng, cfg, key_extra), entry)
     return entry
 
 
-def apply_cached(ce, eng, cfg) -> dict | None:
+def apply_cached(ce, eng, cfg, key_extra: str = "") -> dict | None:
     """Sets the cached best shape on the engine if there is one."""
-    e = load(_key(ce, eng, cfg))
+    e = load(_key(eng, cfg, key_extra))
     if e:
         eng.set_launch(**e["shape"])
         return e["shape"]

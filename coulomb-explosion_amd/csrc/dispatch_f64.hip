@@ -1,0 +1,9 @@
+// f64 integrator kernels (the on-GPU accuracy oracle path).
+#include "dispatch.hpp"
+
+namespace ce {
+hipError_t launch_dp5_f64(const LaunchReq& req, LaunchInfo* info, bool* bad_n) {
+  return req.per_sim_charges ? launch_all<double, true>(req, info, bad_n)
+                             : launch_all<double, false>(req, info, bad_n);
+}
+}  // namespace ce

@@ -65,18 +65,29 @@ constexpr DP5Tables make_dp5_tables() {
   t.c[4] = 8.0 / 9;
   t.c[5] = 1.0;
   t.c[6] = 1.0;
-  for (int s = 0; s < 7; ++s)
-    for (int l = 0; l < 7; ++l) {
-      double acc = 0.0;
-      for (int j = l + 1; j < s; ++j) acc += t.a[s][j] * t.a[j][l];
-      t.aa[s][l] = acc;
-    }
-  for (int l = 0; l < 7; ++l) {
-    t.b[l] = t.a[6][l];
-    double acc = 0.0;
-    for (int s = l + 1; s < 7; ++s) acc += t.e[s] * t.a[s][l];
-    t.ea[l] = acc;
-  }
+  // A*A and E*A evaluated exactly in rational arithmetic from the tableau above
+  // (a double-precision sum leaves rounding residue, e.g. -5.6e-17 for the exact
+  // zero (E*A)[1]).
+  t.aa[2][0] = 9.0 / 200;
+  t.aa[3][0] = -12.0 / 25;
+  t.aa[3][1] = 4.0 / 5;
+  t.aa[4][0] = -12248.0 / 6561;
+  t.aa[4][1] = 7208.0 / 2187;
+  t.aa[4][2] = -6784.0 / 6561;
+  t.aa[5][0] = -533.0 / 264;
+  t.aa[5][1] = 91.0 / 22;
+  t.aa[5][2] = -56.0 / 33;
+  t.aa[5][3] = 7.0 / 88;
+  t.aa[6][0] = 35.0 / 384;
+  t.aa[6][2] = 50.0 / 159;
+  t.aa[6][3] = 25.0 / 192;
+  t.aa[6][4] = -243.0 / 6784;
+  t.ea[0] = 611.0 / 230400;
+  t.ea[2] = -514.0 / 83475;
+  t.ea[3] = 391.0 / 38400;
+  t.ea[4] = -4617.0 / 1356800;
+  t.ea[5] = -11.0 / 3360;
+  for (int l = 0; l < 7; ++l) t.b[l] = t.a[6][l];
   return t;
 }
 

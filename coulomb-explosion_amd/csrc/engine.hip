@@ -1,5 +1,11 @@
 // Host side of the C-ABI (include/coulomb_explosion.h): engine lifetime, device
-// buffers, launch shape, and the per-(precision, N) kernel dispatch.
+// buffers, species / segment layout, launch shape, and the kernel dispatch.
+//
+// An engine holds 1..4 species (molecule types).  Species s owns the global sim
+// range [start_s, start_s + count_s); per-sim arrays (steps, status, ...) are
+// indexed by global sim, per-atom arrays (geometry, momenta) are species-major:
+// species s starts at pos_off_s = sum_{r<s} count_r * 3 n_r.  A single-molecule
+// engine (ce_engine_create) is one species whose count is the loaded batch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -7,9 +13,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "coulomb_explosion.h"
-#include "kernels.hpp"
+#include "dispatch.hpp"
 
 namespace {
 
@@ -20,11 +27,11 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define CE_HIP(call)                                                                       \
-  do {                                                                                     \
-    hipError_t e_ = (call);                                                                \
-    if (e_ != hipSuccess)                                                                  \
-      return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_));          \
+#define CE_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
 template <class T>
@@ -35,6 +42,17 @@ int hip_alloc(T** p, size_t count) {
   return CE_OK;
 }
 
+struct Species {
+  int n = 0;
+  double masses[CE_MAX_ATOMS] = {};
+  double charges[CE_MAX_ATOMS] = {};
+  double radius = 0, min_sep = 0;
+  int levels = 0;      // per-sim charge levels (0 = fixed charges)
+  int max_total = 0;
+  uint64_t capacity = 0;  // sims this species can hold
+  uint64_t count = 0;     // sims loaded for the next run
+};
+
 }  // namespace
 
 namespace ce {
@@ -44,18 +62,21 @@ int set_error(int code, const std::string& msg) { return fail(code, msg); }
 struct ce_engine {
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t k0 = nullptr, k1 = nullptr, s0 = nullptr, s1 = nullptr;
-  int n = 0;
-  double masses[CE_MAX_ATOMS] = {};
-  double charges[CE_MAX_ATOMS] = {};
+  hipEvent_t k0 = nullptr, k1 = nullptr, s0 = nullptr, s1 = nullptr, p0 = nullptr, p1 = nullptr;
+  int n_species = 0;
+  Species sp[CE_MAX_SPECIES];
+  bool mixed = false;        // created by ce_engine_create_mixed (fixed species counts)
   double coulomb_k = 1.0;
   double rtol = 0, atol = 0;
-  size_t capacity = 0;
-  size_t n_loaded = 0;
+  size_t total_pos = 0;      // sum capacity * 3n (doubles)
+  size_t total_q = 0;        // sum capacity * n
+  size_t capacity = 0;       // sum of species capacities
   bool sampled = false;
+  bool have_charges = false; // qsim holds per-sim charges for the loaded sims
   // device buffers
   double* geom = nullptr;
   uint8_t* gstat = nullptr;
+  float* qsim = nullptr;
   unsigned long long* ctr = nullptr;
   double* mom = nullptr;
   float* mom32 = nullptr;
@@ -66,74 +87,107 @@ struct ce_engine {
   double* eerr = nullptr;
   double* trace = nullptr;
   uint32_t trace_cap = 0;
-  uint8_t* records = nullptr;   // device staging of packed records (lazy)
-  size_t records_cap = 0;       // sims it holds
-  hipEvent_t p0 = nullptr, p1 = nullptr;
+  uint8_t* records = nullptr;
+  size_t records_cap = 0;
   // launch
-  ce_launch_config launch = {0, 0, 0, 0};
+  ce_launch_config launch = {0, 0, 4, 64};
   int n_cus = 0;
   ce_run_stats stats = {};
   bool have_results = false;
+
+  size_t loaded() const {
+    size_t t = 0;
+    for (int s = 0; s < n_species; ++s) t += sp[s].count;
+    return t;
+  }
+  // start / offsets of species s for the loaded counts (single species: 0)
+  void offsets(int s, uint64_t* start, uint64_t* pos_off, uint64_t* q_off) const {
+    uint64_t a = 0, b = 0, c = 0;
+    for (int r = 0; r < s; ++r) {
+      a += sp[r].capacity;
+      b += sp[r].capacity * 3 * sp[r].n;
+      c += sp[r].capacity * sp[r].n;
+    }
+    *start = a;
+    *pos_off = b;
+    *q_off = c;
+  }
 };
 
 namespace {
 
-template <class T, int N>
-ce::MolConst<T, N> make_const(const ce_engine* e) {
-  ce::MolConst<T, N> mc{};
-  for (int i = 0; i < N; ++i) {
-    mc.im[i] = static_cast<T>(1.0 / e->masses[i]);
-    mc.md[i] = e->masses[i];
-    for (int j = i + 1; j < N; ++j) {
-      const int p = ce::pair_index<N>(i, j);
-      mc.pcd[p] = e->coulomb_k * e->charges[i] * e->charges[j];
-      mc.pc[p] = static_cast<T>(mc.pcd[p]);
-    }
-  }
-  return mc;
+void free_engine(ce_engine* e) {
+  if (e == nullptr) return;
+  (void)hipSetDevice(e->device);
+  void* bufs[] = {e->geom,   e->gstat,    e->qsim,   e->ctr,  e->mom,  e->mom32,  e->steps,
+                  e->attempts, e->status, e->tfin, e->eerr, e->trace, e->records};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  hipEvent_t evs[] = {e->k0, e->k1, e->s0, e->s1, e->p0, e->p1};
+  for (hipEvent_t v : evs)
+    if (v) (void)hipEventDestroy(v);
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  delete e;
 }
 
-template <class T, int N>
-int launch_dp5(ce_engine* e, const ce::RunArgs& a, int* grid_out, int* block_out) {
-  const auto kern = ce::dp5_persistent<T, N>;
-  const int block = e->launch.block_threads > 0 ? e->launch.block_threads : 256;
-  int per_cu = e->launch.blocks_per_cu;
-  if (per_cu <= 0) {
-    int occ = 0;
-    CE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0));
-    per_cu = std::max(occ, 1);
-  }
-  const int grid = per_cu * e->n_cus;
-  hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, e->stream, a, make_const<T, N>(e));
-  CE_HIP(hipGetLastError());
-  *grid_out = grid;
-  *block_out = block;
+int check_species(const double* masses, int n) {
+  if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms must be in [1, CE_MAX_ATOMS]");
+  for (int i = 0; i < n; ++i)
+    if (!(masses[i] > 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "masses must be positive");
   return CE_OK;
 }
 
-template <class T>
-int dispatch_dp5(ce_engine* e, const ce::RunArgs& a, int* g, int* b) {
-  switch (e->n) {
-    case 1: return launch_dp5<T, 1>(e, a, g, b);
-    case 2: return launch_dp5<T, 2>(e, a, g, b);
-    case 3: return launch_dp5<T, 3>(e, a, g, b);
-    case 4: return launch_dp5<T, 4>(e, a, g, b);
-    case 5: return launch_dp5<T, 5>(e, a, g, b);
-    case 6: return launch_dp5<T, 6>(e, a, g, b);
-    case 7: return launch_dp5<T, 7>(e, a, g, b);
-    case 8: return launch_dp5<T, 8>(e, a, g, b);
-    case 9: return launch_dp5<T, 9>(e, a, g, b);
-    case 10: return launch_dp5<T, 10>(e, a, g, b);
-    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+// Allocates the engine's device state for its species (capacities set).
+int init_engine(ce_engine* e, const ce_tolerances* tol) {
+  if (!(tol->rtol > 0.0) || !(tol->atol >= 0.0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "rtol must be > 0 and atol >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(CE_ERR_NO_DEVICE, "no HIP device");
+  CE_HIP(hipGetDevice(&e->device));
+  e->rtol = tol->rtol;
+  e->atol = tol->atol;
+  e->capacity = 0;
+  e->total_pos = 0;
+  e->total_q = 0;
+  for (int s = 0; s < e->n_species; ++s) {
+    e->capacity += e->sp[s].capacity;
+    e->total_pos += e->sp[s].capacity * 3 * e->sp[s].n;
+    e->total_q += e->sp[s].capacity * e->sp[s].n;
   }
+  hipDeviceProp_t prop;
+  CE_HIP(hipGetDeviceProperties(&prop, e->device));
+  e->n_cus = prop.multiProcessorCount;
+  e->stats.n_cus = e->n_cus;
+  CE_HIP(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+  hipEvent_t* evs[] = {&e->k0, &e->k1, &e->s0, &e->s1, &e->p0, &e->p1};
+  for (hipEvent_t* v : evs) CE_HIP(hipEventCreate(v));
+  int rc = CE_OK;
+  if (rc == CE_OK) rc = hip_alloc(&e->geom, e->total_pos);
+  if (rc == CE_OK) rc = hip_alloc(&e->gstat, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->ctr, ce::kCtrWords);
+  if (rc == CE_OK) rc = hip_alloc(&e->mom, e->total_pos);
+  if (rc == CE_OK) rc = hip_alloc(&e->mom32, e->total_pos);
+  if (rc == CE_OK) rc = hip_alloc(&e->steps, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->attempts, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->status, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->tfin, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->eerr, e->capacity);
+  bool any_levels = false;
+  for (int s = 0; s < e->n_species; ++s) any_levels = any_levels || e->sp[s].levels > 0;
+  if (rc == CE_OK && any_levels) rc = hip_alloc(&e->qsim, e->total_q);
+  return rc;
 }
 
-template <int N>
-void launch_sample(const ce::SampleArgs& a, hipStream_t s) {
-  const int block = 256;
-  const uint64_t blocks = (a.n_sims + block - 1) / block;
-  const int grid = static_cast<int>(std::min<uint64_t>(blocks, 65536));
-  hipLaunchKernelGGL(ce::sample_sphere<N>, dim3(grid), dim3(block), 0, s, a);
+int finish_create(ce_engine* e, int rc, ce_engine** out) {
+  if (rc != CE_OK) {
+    const std::string msg = g_err;
+    free_engine(e);
+    g_err = msg;
+    return rc;
+  }
+  *out = e;
+  return CE_OK;
 }
 
 int check_config(const ce_run_config* c) {
@@ -153,44 +207,127 @@ int check_config(const ce_run_config* c) {
   return CE_OK;
 }
 
+// Species-major per-atom arrays are packed at the *capacity* offsets on the
+// device; the host sees them packed at the *loaded* counts.  For a
+// single-species engine the two coincide.
+int copy_per_atom(ce_engine* e, void* host, const void* dev, size_t elem, int per_atom_width,
+                  hipMemcpyKind kind) {
+  size_t host_off = 0;
+  for (int s = 0; s < e->n_species; ++s) {
+    uint64_t start, pos_off, q_off;
+    e->offsets(s, &start, &pos_off, &q_off);
+    const size_t dev_off = per_atom_width == 3 ? pos_off : q_off;
+    const size_t cnt = e->sp[s].count * per_atom_width * e->sp[s].n;
+    if (cnt == 0) continue;
+    char* h = static_cast<char*>(host) + host_off * elem;
+    const char* d = static_cast<const char*>(dev) + dev_off * elem;
+    if (kind == hipMemcpyDeviceToHost)
+      CE_HIP(hipMemcpyAsync(h, d, cnt * elem, kind, e->stream));
+    else
+      CE_HIP(hipMemcpyAsync(const_cast<char*>(d), h, cnt * elem, kind, e->stream));
+    host_off += cnt;
+  }
+  return CE_OK;
+}
+
+// Per-sim arrays: the device keeps species s at global index start_s (capacity
+// based); the host sees the loaded sims back to back.
+int copy_per_sim(ce_engine* e, void* host, const void* dev, size_t elem, hipMemcpyKind kind) {
+  size_t host_off = 0;
+  for (int s = 0; s < e->n_species; ++s) {
+    uint64_t start, pos_off, q_off;
+    e->offsets(s, &start, &pos_off, &q_off);
+    const size_t cnt = e->sp[s].count;
+    if (cnt == 0) continue;
+    char* h = static_cast<char*>(host) + host_off * elem;
+    const char* d = static_cast<const char*>(dev) + start * elem;
+    if (kind == hipMemcpyDeviceToHost)
+      CE_HIP(hipMemcpyAsync(h, d, cnt * elem, kind, e->stream));
+    else
+      CE_HIP(hipMemcpyAsync(const_cast<char*>(d), h, cnt * elem, kind, e->stream));
+    host_off += cnt;
+  }
+  return CE_OK;
+}
+
 int copy_results(ce_engine* e, ce_results* out) {
   if (out == nullptr) return CE_OK;
   if (!e->have_results) return fail(CE_ERR_RUNTIME, "no results: run the engine first");
-  const size_t ns = e->n_loaded, n3 = 3 * static_cast<size_t>(e->n);
-  auto d2h = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
-    return dst ? hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, e->stream) : hipSuccess;
-  };
-  CE_HIP(d2h(out->momenta, e->mom, ns * n3 * sizeof(double)));
-  CE_HIP(d2h(out->momenta_f32, e->mom32, ns * n3 * sizeof(float)));
-  CE_HIP(d2h(out->steps, e->steps, ns * sizeof(uint32_t)));
-  CE_HIP(d2h(out->attempts, e->attempts, ns * sizeof(uint32_t)));
-  CE_HIP(d2h(out->status, e->status, ns));
-  CE_HIP(d2h(out->t_final, e->tfin, ns * sizeof(double)));
-  CE_HIP(d2h(out->energy_err, e->eerr, ns * sizeof(double)));
-  if (out->trace_h != nullptr) {
+  int rc = CE_OK;
+  const hipMemcpyKind d2h = hipMemcpyDeviceToHost;
+  if (rc == CE_OK && out->momenta) rc = copy_per_atom(e, out->momenta, e->mom, sizeof(double), 3, d2h);
+  if (rc == CE_OK && out->momenta_f32) rc = copy_per_atom(e, out->momenta_f32, e->mom32, sizeof(float), 3, d2h);
+  if (rc == CE_OK && out->steps) rc = copy_per_sim(e, out->steps, e->steps, sizeof(uint32_t), d2h);
+  if (rc == CE_OK && out->attempts) rc = copy_per_sim(e, out->attempts, e->attempts, sizeof(uint32_t), d2h);
+  if (rc == CE_OK && out->status) rc = copy_per_sim(e, out->status, e->status, 1, d2h);
+  if (rc == CE_OK && out->t_final) rc = copy_per_sim(e, out->t_final, e->tfin, sizeof(double), d2h);
+  if (rc == CE_OK && out->energy_err) rc = copy_per_sim(e, out->energy_err, e->eerr, sizeof(double), d2h);
+  if (rc == CE_OK && out->trace_h != nullptr) {
     if (out->trace_cap != e->trace_cap || e->trace == nullptr)
       return fail(CE_ERR_INVALID_ARGUMENT, "trace_cap does not match the enabled trace");
-    CE_HIP(d2h(out->trace_h, e->trace, ns * e->trace_cap * sizeof(double)));
+    rc = copy_per_sim(e, out->trace_h, e->trace, sizeof(double) * e->trace_cap, d2h);
   }
+  if (rc != CE_OK) return rc;
   CE_HIP(hipStreamSynchronize(e->stream));
   return CE_OK;
 }
 
-void free_engine(ce_engine* e) {
-  if (e == nullptr) return;
-  hipSetDevice(e->device);
-  void* bufs[] = {e->geom,     e->gstat,  e->ctr,  e->mom,  e->mom32,  e->steps,
-                  e->attempts, e->status, e->tfin, e->eerr, e->trace, e->records};
-  for (void* b : bufs)
-    if (b) hipFree(b);
-  if (e->k0) hipEventDestroy(e->k0);
-  if (e->k1) hipEventDestroy(e->k1);
-  if (e->s0) hipEventDestroy(e->s0);
-  if (e->s1) hipEventDestroy(e->s1);
-  if (e->p0) hipEventDestroy(e->p0);
-  if (e->p1) hipEventDestroy(e->p1);
-  if (e->stream) hipStreamDestroy(e->stream);
-  delete e;
+template <int N>
+void launch_sample(const ce::SampleArgs& a, hipStream_t s) {
+  const int block = 256;
+  const uint64_t blocks = (a.n_sims + block - 1) / block;
+  const int grid = static_cast<int>(std::min<uint64_t>(blocks, 65536));
+  hipLaunchKernelGGL(ce::sample_sphere<N>, dim3(grid), dim3(block), 0, s, a);
+}
+
+int sample_segment(ce_engine* e, int s, uint64_t seed, uint64_t first_global, double radius,
+                   double min_sep, uint32_t max_attempts, bool draw_charges) {
+  uint64_t start, pos_off, q_off;
+  e->offsets(s, &start, &pos_off, &q_off);
+  ce::SampleArgs a{};
+  a.geom = e->geom + pos_off;
+  a.status = e->gstat + start;
+  a.charges = draw_charges ? e->qsim + q_off : nullptr;
+  a.first_index = first_global;
+  a.n_sims = e->sp[s].count;
+  a.key0 = static_cast<uint32_t>(seed);
+  a.key1 = static_cast<uint32_t>(seed >> 32);
+  a.R = static_cast<float>(radius);
+  a.twoR = static_cast<float>(2.0 * radius);
+  a.R2 = a.R * a.R;
+  const float ms = static_cast<float>(min_sep);
+  a.min2 = ms * ms;
+  a.max_attempts = max_attempts;
+  a.n_atoms = e->sp[s].n;
+  a.charge_levels = e->sp[s].levels;
+  a.max_total_charge = e->sp[s].max_total;
+  if (a.n_sims == 0) return CE_OK;
+  switch (e->sp[s].n) {
+    case 1: launch_sample<1>(a, e->stream); break;
+    case 2: launch_sample<2>(a, e->stream); break;
+    case 3: launch_sample<3>(a, e->stream); break;
+    case 4: launch_sample<4>(a, e->stream); break;
+    case 5: launch_sample<5>(a, e->stream); break;
+    case 6: launch_sample<6>(a, e->stream); break;
+    case 7: launch_sample<7>(a, e->stream); break;
+    case 8: launch_sample<8>(a, e->stream); break;
+    case 9: launch_sample<9>(a, e->stream); break;
+    case 10: launch_sample<10>(a, e->stream); break;
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CE_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+int finish_sampling(ce_engine* e) {
+  CE_HIP(hipEventRecord(e->s1, e->stream));
+  CE_HIP(hipStreamSynchronize(e->stream));
+  float ms = 0.0f;
+  CE_HIP(hipEventElapsedTime(&ms, e->s0, e->s1));
+  e->stats.sampler_ms = ms;
+  e->sampled = true;
+  e->have_results = false;
+  return CE_OK;
 }
 
 }  // namespace
@@ -235,63 +372,71 @@ int ce_engine_create(const ce_molecule_desc* mol, const ce_tolerances* tol, size
     return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms must be in [1, CE_MAX_ATOMS]");
   if (mol->masses == nullptr || mol->charges == nullptr)
     return fail(CE_ERR_INVALID_ARGUMENT, "masses/charges NULL");
-  for (int i = 0; i < mol->n_atoms; ++i)
-    if (!(mol->masses[i] > 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "masses must be positive");
+  int rc = check_species(mol->masses, mol->n_atoms);
+  if (rc != CE_OK) return rc;
   if (!(tol->rtol > 0.0) || !(tol->atol >= 0.0))
     return fail(CE_ERR_INVALID_ARGUMENT, "rtol must be > 0 and atol >= 0");
   if (capacity == 0) return fail(CE_ERR_INVALID_ARGUMENT, "capacity must be positive");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     return fail(CE_ERR_NO_DEVICE, "no HIP device");
-
   ce_engine* e = new ce_engine();
-  CE_HIP(hipGetDevice(&e->device));
-  e->n = mol->n_atoms;
-  for (int i = 0; i < e->n; ++i) {
-    e->masses[i] = mol->masses[i];
-    e->charges[i] = mol->charges[i];
+  e->n_species = 1;
+  Species& s = e->sp[0];
+  s.n = mol->n_atoms;
+  for (int i = 0; i < s.n; ++i) {
+    s.masses[i] = mol->masses[i];
+    s.charges[i] = mol->charges[i];
   }
+  s.capacity = capacity;
   e->coulomb_k = mol->coulomb_k;
-  e->rtol = tol->rtol;
-  e->atol = tol->atol;
-  e->capacity = capacity;
-  hipDeviceProp_t prop;
-  int rc = CE_OK;
-  auto chk = [&](hipError_t h, const char* what) {
-    if (h != hipSuccess && rc == CE_OK)
-      rc = fail(CE_ERR_HIP, std::string(what) + ": " + hipGetErrorString(h));
-  };
-  chk(hipGetDeviceProperties(&prop, e->device), "hipGetDeviceProperties");
-  if (rc == CE_OK) e->n_cus = prop.multiProcessorCount;
-  e->stats.n_cus = e->n_cus;
-  chk(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "hipStreamCreate");
-  chk(hipEventCreate(&e->k0), "hipEventCreate");
-  chk(hipEventCreate(&e->k1), "hipEventCreate");
-  chk(hipEventCreate(&e->s0), "hipEventCreate");
-  chk(hipEventCreate(&e->s1), "hipEventCreate");
-  chk(hipEventCreate(&e->p0), "hipEventCreate");
-  chk(hipEventCreate(&e->p1), "hipEventCreate");
-  const size_t n3 = 3 * static_cast<size_t>(e->n);
-  if (rc == CE_OK) rc = hip_alloc(&e->geom, capacity * n3);
-  if (rc == CE_OK) rc = hip_alloc(&e->gstat, capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->ctr, 8);
-  if (rc == CE_OK) rc = hip_alloc(&e->mom, capacity * n3);
-  if (rc == CE_OK) rc = hip_alloc(&e->mom32, capacity * n3);
-  if (rc == CE_OK) rc = hip_alloc(&e->steps, capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->attempts, capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->status, capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->tfin, capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->eerr, capacity);
-  if (rc != CE_OK) {
-    const std::string msg = g_err;
-    free_engine(e);
-    g_err = msg;
-    return rc;
+  return finish_create(e, init_engine(e, tol), out);
+}
+
+int ce_engine_create_mixed(const ce_species_desc* species, int32_t n_species, double coulomb_k,
+                           const ce_tolerances* tol, ce_engine** out) {
+  if (out == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (species == nullptr || tol == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n_species < 1 || n_species > CE_MAX_SPECIES)
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_species must be in [1, CE_MAX_SPECIES]");
+  for (int s = 0; s < n_species; ++s) {
+    const ce_species_desc& d = species[s];
+    if (d.masses == nullptr || (d.charges == nullptr && d.charge_levels == 0))
+      return fail(CE_ERR_INVALID_ARGUMENT, "masses/charges NULL");
+    const int rc = check_species(d.masses, d.n_atoms);
+    if (rc != CE_OK) return rc;
+    if (d.charge_levels < 0 || d.charge_levels > 255)
+      return fail(CE_ERR_INVALID_ARGUMENT, "charge_levels must be in [0, 255]");
+    if (d.count == 0) return fail(CE_ERR_INVALID_ARGUMENT, "species count must be positive");
+    if (!(d.radius > 0.0) || !(d.min_separation >= 0.0))
+      return fail(CE_ERR_INVALID_ARGUMENT, "species radius must be > 0, min_separation >= 0");
   }
-  e->launch.refill_threshold = 4;
-  e->launch.chunk = 64;
-  *out = e;
-  return CE_OK;
+  if (!(tol->rtol > 0.0) || !(tol->atol >= 0.0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "rtol must be > 0 and atol >= 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(CE_ERR_NO_DEVICE, "no HIP device");
+  ce_engine* e = new ce_engine();
+  e->n_species = n_species;
+  e->mixed = true;
+  e->coulomb_k = coulomb_k;
+  for (int s = 0; s < n_species; ++s) {
+    const ce_species_desc& d = species[s];
+    Species& sp = e->sp[s];
+    sp.n = d.n_atoms;
+    for (int i = 0; i < sp.n; ++i) {
+      sp.masses[i] = d.masses[i];
+      sp.charges[i] = d.charges ? d.charges[i] : 1.0;
+    }
+    sp.radius = d.radius;
+    sp.min_sep = d.min_separation;
+    sp.levels = d.charge_levels;
+    sp.max_total = d.max_total_charge;
+    sp.capacity = d.count;
+    sp.count = d.count;
+  }
+  return finish_create(e, init_engine(e, tol), out);
 }
 
 int ce_engine_destroy(ce_engine* e) {
@@ -302,73 +447,120 @@ int ce_engine_destroy(ce_engine* e) {
 int ce_engine_set_geometries(ce_engine* e, const double* pos, size_t n_sims) {
   if (e == nullptr || (pos == nullptr && n_sims > 0))
     return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (n_sims > e->capacity) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds capacity");
+  if (e->mixed) {
+    if (n_sims != e->capacity)
+      return fail(CE_ERR_INVALID_ARGUMENT, "a mixed engine takes all its sims at once");
+  } else {
+    if (n_sims > e->capacity) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds capacity");
+    e->sp[0].count = n_sims;
+  }
   CE_HIP(hipSetDevice(e->device));
-  const size_t bytes = n_sims * 3 * static_cast<size_t>(e->n) * sizeof(double);
-  if (bytes) CE_HIP(hipMemcpyAsync(e->geom, pos, bytes, hipMemcpyHostToDevice, e->stream));
+  int rc = copy_per_atom(e, const_cast<double*>(pos), e->geom, sizeof(double), 3,
+                         hipMemcpyHostToDevice);
+  if (rc != CE_OK) return rc;
   CE_HIP(hipStreamSynchronize(e->stream));
-  e->n_loaded = n_sims;
   e->sampled = false;
+  e->have_charges = false;
   e->have_results = false;
+  return CE_OK;
+}
+
+int ce_engine_set_charges(ce_engine* e, const double* charges, size_t n_sims) {
+  if (e == nullptr || (charges == nullptr && n_sims > 0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n_sims != e->loaded())
+    return fail(CE_ERR_INVALID_ARGUMENT, "charges must cover exactly the loaded sims");
+  CE_HIP(hipSetDevice(e->device));
+  if (e->qsim == nullptr) {
+    const int rc = hip_alloc(&e->qsim, e->total_q);
+    if (rc != CE_OK) return rc;
+  }
+  size_t total = 0;
+  for (int s = 0; s < e->n_species; ++s) total += e->sp[s].count * e->sp[s].n;
+  std::vector<float> q(total);
+  for (size_t i = 0; i < total; ++i) q[i] = static_cast<float>(charges[i]);
+  int rc = copy_per_atom(e, q.data(), e->qsim, sizeof(float), 1, hipMemcpyHostToDevice);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipStreamSynchronize(e->stream));
+  e->have_charges = true;
+  e->have_results = false;
+  return CE_OK;
+}
+
+int ce_engine_get_charges(ce_engine* e, double* charges, size_t n_sims) {
+  if (e == nullptr || charges == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!e->have_charges) return fail(CE_ERR_RUNTIME, "the loaded sims carry no per-sim charges");
+  if (n_sims != e->loaded())
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_sims must equal the loaded sims");
+  CE_HIP(hipSetDevice(e->device));
+  size_t total = 0;
+  for (int s = 0; s < e->n_species; ++s) total += e->sp[s].count * e->sp[s].n;
+  std::vector<float> q(total);
+  int rc = copy_per_atom(e, q.data(), e->qsim, sizeof(float), 1, hipMemcpyDeviceToHost);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipStreamSynchronize(e->stream));
+  for (size_t i = 0; i < total; ++i) charges[i] = q[i];
   return CE_OK;
 }
 
 int ce_engine_sample(ce_engine* e, uint64_t seed, uint64_t first_index, size_t n_sims,
                      double radius, double min_sep, uint32_t max_attempts) {
   if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (e->mixed) return fail(CE_ERR_INVALID_ARGUMENT, "mixed engines sample with ce_engine_sample_species");
   if (!(radius > 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "radius must be positive");
   if (!(min_sep >= 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "min_separation must be >= 0");
   if (max_attempts == 0) return fail(CE_ERR_INVALID_ARGUMENT, "max_attempts must be positive");
   if (n_sims > e->capacity) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds capacity");
   CE_HIP(hipSetDevice(e->device));
-  ce::SampleArgs a{};
-  a.geom = e->geom;
-  a.status = e->gstat;
-  a.first_index = first_index;
-  a.n_sims = n_sims;
-  a.key0 = static_cast<uint32_t>(seed);
-  a.key1 = static_cast<uint32_t>(seed >> 32);
-  a.R = static_cast<float>(radius);
-  a.twoR = static_cast<float>(2.0 * radius);
-  a.R2 = a.R * a.R;
-  const float ms = static_cast<float>(min_sep);
-  a.min2 = ms * ms;
-  a.max_attempts = max_attempts;
-  a.n_atoms = e->n;
+  e->sp[0].count = n_sims;
   CE_HIP(hipEventRecord(e->s0, e->stream));
-  if (n_sims > 0) {
-    switch (e->n) {
-      case 1: launch_sample<1>(a, e->stream); break;
-      case 2: launch_sample<2>(a, e->stream); break;
-      case 3: launch_sample<3>(a, e->stream); break;
-      case 4: launch_sample<4>(a, e->stream); break;
-      case 5: launch_sample<5>(a, e->stream); break;
-      case 6: launch_sample<6>(a, e->stream); break;
-      case 7: launch_sample<7>(a, e->stream); break;
-      case 8: launch_sample<8>(a, e->stream); break;
-      case 9: launch_sample<9>(a, e->stream); break;
-      case 10: launch_sample<10>(a, e->stream); break;
-      default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
-    }
-    CE_HIP(hipGetLastError());
+  const int rc = sample_segment(e, 0, seed, first_index, radius, min_sep, max_attempts, false);
+  if (rc != CE_OK) return rc;
+  e->have_charges = false;
+  return finish_sampling(e);
+}
+
+int ce_engine_sample_species(ce_engine* e, uint64_t seed, uint64_t first_index,
+                             uint32_t max_attempts) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (!e->mixed) return fail(CE_ERR_INVALID_ARGUMENT, "engine has no species table");
+  if (max_attempts == 0) return fail(CE_ERR_INVALID_ARGUMENT, "max_attempts must be positive");
+  CE_HIP(hipSetDevice(e->device));
+  CE_HIP(hipEventRecord(e->s0, e->stream));
+  bool any_levels = false;
+  for (int s = 0; s < e->n_species; ++s) {
+    uint64_t start, pos_off, q_off;
+    e->offsets(s, &start, &pos_off, &q_off);
+    const Species& sp = e->sp[s];
+    const int rc = sample_segment(e, s, seed, first_index + start, sp.radius, sp.min_sep,
+                                  max_attempts, sp.levels > 0);
+    if (rc != CE_OK) return rc;
+    any_levels = any_levels || sp.levels > 0;
   }
-  CE_HIP(hipEventRecord(e->s1, e->stream));
-  CE_HIP(hipStreamSynchronize(e->stream));
-  float ms_elapsed = 0.0f;
-  CE_HIP(hipEventElapsedTime(&ms_elapsed, e->s0, e->s1));
-  e->stats.sampler_ms = ms_elapsed;
-  e->n_loaded = n_sims;
-  e->sampled = true;
-  e->have_results = false;
-  return CE_OK;
+  if (any_levels) {
+    // species with fixed charges still need their charges in the per-sim table
+    for (int s = 0; s < e->n_species; ++s) {
+      const Species& sp = e->sp[s];
+      if (sp.levels > 0) continue;
+      uint64_t start, pos_off, q_off;
+      e->offsets(s, &start, &pos_off, &q_off);
+      std::vector<float> q(sp.count * sp.n);
+      for (size_t i = 0; i < q.size(); ++i) q[i] = static_cast<float>(sp.charges[i % sp.n]);
+      CE_HIP(hipMemcpyAsync(e->qsim + q_off, q.data(), q.size() * sizeof(float),
+                            hipMemcpyHostToDevice, e->stream));
+      CE_HIP(hipStreamSynchronize(e->stream));
+    }
+  }
+  e->have_charges = any_levels;
+  return finish_sampling(e);
 }
 
 int ce_engine_get_geometries(ce_engine* e, double* pos, size_t n_sims) {
   if (e == nullptr || pos == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (n_sims > e->n_loaded) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds loaded sims");
+  if (n_sims != e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims must equal the loaded sims");
   CE_HIP(hipSetDevice(e->device));
-  CE_HIP(hipMemcpyAsync(pos, e->geom, n_sims * 3 * static_cast<size_t>(e->n) * sizeof(double),
-                        hipMemcpyDeviceToHost, e->stream));
+  const int rc = copy_per_atom(e, pos, e->geom, sizeof(double), 3, hipMemcpyDeviceToHost);
+  if (rc != CE_OK) return rc;
   CE_HIP(hipStreamSynchronize(e->stream));
   return CE_OK;
 }
@@ -415,10 +607,11 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   int rc = check_config(cfg);
   if (rc != CE_OK) return rc;
   CE_HIP(hipSetDevice(e->device));
-  ce::RunArgs a{};
+  ce::LaunchReq req{};
+  ce::RunArgs& a = req.args;
   a.geom = e->geom;
   a.geom_status = e->sampled ? e->gstat : nullptr;
-  a.n_sims = e->n_loaded;
+  a.qsim = e->have_charges ? e->qsim : nullptr;
   a.ctr = e->ctr;
   a.mom = e->mom;
   a.mom32 = e->mom32;
@@ -437,37 +630,61 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   a.pe_stop_fraction = cfg->pe_stop_fraction;
   a.momentum_tol = cfg->momentum_tol;
   a.h0 = cfg->h0;
+  a.coulomb_k = e->coulomb_k;
   a.max_steps = cfg->max_steps;
   a.redistribute = cfg->redistribute_energy ? 1 : 0;
   a.h0_mode = cfg->h0_mode;
   a.max_rej = cfg->max_rejections;
   a.refill_threshold = std::max(1, std::min(64, e->launch.refill_threshold));
   a.chunk = std::max(64, e->launch.chunk);
-  // Clear the queue head and totals with a kernel (no memset node, so the
+  a.n_segments = e->n_species;
+  a.queue_slot = 0;
+  for (int s = 0; s < e->n_species; ++s) {
+    uint64_t start, pos_off, q_off;
+    e->offsets(s, &start, &pos_off, &q_off);
+    a.seg[s] = ce::Segment{start, e->sp[s].count, pos_off, q_off, e->sp[s].n,
+                           e->have_charges ? 1 : 0};
+    req.species[s].n = e->sp[s].n;
+    for (int i = 0; i < e->sp[s].n; ++i) {
+      req.species[s].masses[i] = e->sp[s].masses[i];
+      req.species[s].charges[i] = e->sp[s].charges[i];
+    }
+  }
+  req.coulomb_k = e->coulomb_k;
+  req.per_sim_charges = e->have_charges;
+  req.block_threads = e->launch.block_threads;
+  req.blocks_per_cu = e->launch.blocks_per_cu;
+  req.n_cus = e->n_cus;
+  req.stream = e->stream;
+  // Clear the queue heads and totals with a kernel (no memset node, so the
   // sequence stays valid inside a captured graph).
-  hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, e->stream, e->ctr, 8);
+  hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, e->stream, e->ctr, ce::kCtrWords);
   CE_HIP(hipGetLastError());
   CE_HIP(hipEventRecord(e->k0, e->stream));
-  int grid = 0, block = 0;
-  if (e->n_loaded > 0) {
-    rc = (cfg->precision == CE_PRECISION_F64) ? dispatch_dp5<double>(e, a, &grid, &block)
-                                              : dispatch_dp5<float>(e, a, &grid, &block);
-    if (rc != CE_OK) return rc;
+  ce::LaunchInfo info;
+  bool bad_n = false;
+  if (e->loaded() > 0) {
+    const hipError_t he = (cfg->precision == CE_PRECISION_F64)
+                              ? ce::launch_dp5_f64(req, &info, &bad_n)
+                              : ce::launch_dp5_f32(req, &info, &bad_n);
+    if (bad_n) return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+    if (he != hipSuccess) return fail(CE_ERR_HIP, std::string("dp5 launch: ") + hipGetErrorString(he));
   }
   CE_HIP(hipEventRecord(e->k1, e->stream));
-  unsigned long long host_ctr[8] = {};
+  unsigned long long host_ctr[ce::kCtrWords] = {};
   CE_HIP(hipMemcpyAsync(host_ctr, e->ctr, sizeof(host_ctr), hipMemcpyDeviceToHost, e->stream));
   CE_HIP(hipStreamSynchronize(e->stream));
   float ms = 0.0f;
   CE_HIP(hipEventElapsedTime(&ms, e->k0, e->k1));
   e->stats.kernel_ms = ms;
-  e->stats.sims = e->n_loaded;
-  e->stats.attempted_steps = host_ctr[1];
-  e->stats.accepted_steps = host_ctr[2];
-  e->stats.refills = host_ctr[3];
-  e->stats.grid_blocks = grid;
-  e->stats.block_threads = block;
+  e->stats.sims = e->loaded();
+  e->stats.attempted_steps = host_ctr[0];
+  e->stats.accepted_steps = host_ctr[1];
+  e->stats.refills = host_ctr[2];
+  e->stats.grid_blocks = info.grid;
+  e->stats.block_threads = info.block;
   e->stats.n_cus = e->n_cus;
+  e->stats.launches = info.launches;
   e->have_results = true;
   return copy_results(e, out);
 }
@@ -491,12 +708,15 @@ size_t ce_record_bytes(int32_t n_atoms) {
 int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t* host_out) {
   if (e == nullptr || (host_out == nullptr && n_sims > 0))
     return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (e->n_species != 1)
+    return fail(CE_ERR_INVALID_ARGUMENT, "records are written per molecule: single-species engines only");
   if (!e->have_results) return fail(CE_ERR_RUNTIME, "no results: run the engine first");
-  if (first > e->n_loaded || n_sims > e->n_loaded - first)
+  if (first > e->loaded() || n_sims > e->loaded() - first)
     return fail(CE_ERR_INVALID_ARGUMENT, "record range exceeds the loaded sims");
   if (n_sims == 0) return CE_OK;
   CE_HIP(hipSetDevice(e->device));
-  const size_t rb = ce_record_bytes(e->n);
+  const int n = e->sp[0].n;
+  const size_t rb = ce_record_bytes(n);
   if (e->records == nullptr || e->records_cap < n_sims) {
     if (e->records) CE_HIP(hipFree(e->records));
     e->records = nullptr;
@@ -508,7 +728,7 @@ int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t*
   const dim3 grid(static_cast<unsigned>((n_sims + ce::kRecBlock - 1) / ce::kRecBlock));
   const dim3 block(ce::kRecBlock);
   CE_HIP(hipEventRecord(e->p0, e->stream));
-  switch (e->n) {
+  switch (n) {
 #define CE_PACK(NN)                                                                            \
   case NN:                                                                                     \
     hipLaunchKernelGGL(ce::pack_records<NN>, grid, block, 0, e->stream, e->geom, e->mom32,     \
@@ -537,7 +757,7 @@ int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const floa
     return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (k == 0) return CE_OK;
   const size_t nk = n * k;
-  float *d = nullptr;
+  float* d = nullptr;
   const size_t total = 6 * nk + n * n + n;
   CE_HIP(hipMalloc(&d, total * sizeof(float)));
   float *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dax = d + 3 * nk, *day = d + 4 * nk,
@@ -554,8 +774,11 @@ int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const floa
   if (rc == CE_OK) {
     const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
     switch (n) {
-#define CE_ACC(NN) \
-  case NN: hipLaunchKernelGGL(ce::soa_accel<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dim, dax, day, daz); break;
+#define CE_ACC(NN)                                                                        \
+  case NN:                                                                                \
+    hipLaunchKernelGGL(ce::soa_accel<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dim, \
+                       dax, day, daz);                                                    \
+    break;
       CE_ACC(1) CE_ACC(2) CE_ACC(3) CE_ACC(4) CE_ACC(5) CE_ACC(6) CE_ACC(7) CE_ACC(8) CE_ACC(9)
       CE_ACC(10)
 #undef CE_ACC
@@ -565,7 +788,7 @@ int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const floa
     chk(hipMemcpy(ay, day, nk * sizeof(float), hipMemcpyDeviceToHost));
     chk(hipMemcpy(az, daz, nk * sizeof(float), hipMemcpyDeviceToHost));
   }
-  hipFree(d);
+  (void)hipFree(d);
   return rc;
 }
 
@@ -589,15 +812,17 @@ int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const f
   if (rc == CE_OK) {
     const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
     switch (n) {
-#define CE_PE(NN) \
-  case NN: hipLaunchKernelGGL(ce::soa_pe<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dpe); break;
+#define CE_PE(NN)                                                                              \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL(ce::soa_pe<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dpe); \
+    break;
       CE_PE(1) CE_PE(2) CE_PE(3) CE_PE(4) CE_PE(5) CE_PE(6) CE_PE(7) CE_PE(8) CE_PE(9) CE_PE(10)
 #undef CE_PE
     }
     chk(hipGetLastError());
     chk(hipMemcpy(pe, dpe, k * sizeof(float), hipMemcpyDeviceToHost));
   }
-  hipFree(d);
+  (void)hipFree(d);
   return rc;
 }
 

@@ -37,6 +37,12 @@ __device__ __forceinline__ double rsqrtT(double a) { return 1.0 / __builtin_sqrt
 // 1/x inside the error norm: v_rcp_f32 (1 ulp) / exact in f64.
 __device__ __forceinline__ float rcpT(float a) { return __builtin_amdgcn_rcpf(a); }
 __device__ __forceinline__ double rcpT(double a) { return 1.0 / a; }
+// 2^x and log2(x): v_exp_f32 / v_log_f32 (1 ulp) on the f32 path, the OCML
+// functions on the f64 path.
+__device__ __forceinline__ float exp2T(float a) { return __builtin_amdgcn_exp2f(a); }
+__device__ __forceinline__ double exp2T(double a) { return exp2(a); }
+__device__ __forceinline__ float log2T(float a) { return __builtin_amdgcn_logf(a); }
+__device__ __forceinline__ double log2T(double a) { return log2(a); }
 // std::min / std::max semantics ((b < a) ? b : a), NaN in b keeps a.
 template <class T>
 __device__ __forceinline__ T minT(T a, T b) { return (b < a) ? b : a; }
@@ -72,12 +78,35 @@ constexpr int pair_index(int i, int j) {  // i < j
   return i * N - i * (i + 1) / 2 + (j - i - 1);
 }
 
+// Pair constant k q_i q_j of pair (i, j): from the molecule table, or -- when
+// every simulation draws its own charges (PS) -- from the lane's charges.
+template <class T, int N, bool PS>
+struct PairK {
+  const MolConst<T, N>& mc;
+  T kq[PS ? N : 1];  // k q_i
+  T q[PS ? N : 1];   // q_j
+  template <int I, int J>
+  __device__ __forceinline__ T get() const {
+    if constexpr (PS)
+      return kq[I] * q[J];
+    else
+      return mc.pc[pair_index<N>(I, J)];
+  }
+  template <int I, int J>
+  __device__ __forceinline__ double getd() const {
+    if constexpr (PS)
+      return static_cast<double>(kq[I]) * static_cast<double>(q[J]);
+    else
+      return mc.pcd[pair_index<N>(I, J)];
+  }
+};
+
 // Accelerations a(x) of one lane (x, a: [3N] atom-major), optionally the PE.
 // Pair term: r = x_i - x_j, d2 = r.r (fma), inv = 1/|r|, s = k q_i q_j inv^3,
 // f = r s accumulated +f on i, -f on j (Newton 3), then a_i *= 1/m_i.
-template <class T, int N, bool WITH_PE>
+template <class T, int N, bool PS, bool WITH_PE, bool WITH_IM = true>
 __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T& pe,
-                                           const MolConst<T, N>& mc) {
+                                           const PairK<T, N, PS>& pk) {
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) a[q] = T(0);
   T u = T(0);
@@ -85,15 +114,15 @@ __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T
     constexpr int i = decltype(I)::value;
     static_for<i + 1, N>([&](auto J) {
       constexpr int j = decltype(J)::value;
-      constexpr int p = pair_index<N>(i, j);
+      const T c = pk.template get<i, j>();
       const T rx = x[3 * i] - x[3 * j];
       const T ry = x[3 * i + 1] - x[3 * j + 1];
       const T rz = x[3 * i + 2] - x[3 * j + 2];
       const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
       const T inv = rsqrtT(d2);
       const T inv3 = (inv * inv) * inv;
-      const T s = mc.pc[p] * inv3;
-      if constexpr (WITH_PE) u = fmaT(mc.pc[p], inv, u);
+      const T s = c * inv3;
+      if constexpr (WITH_PE) u = fmaT(c, inv, u);
       const T fx = rx * s, fy = ry * s, fz = rz * s;
       a[3 * i] += fx;
       a[3 * i + 1] += fy;
@@ -103,17 +132,19 @@ __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T
       a[3 * j + 2] -= fz;
     });
   });
+  if constexpr (WITH_IM) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    a[3 * i] *= mc.im[i];
-    a[3 * i + 1] *= mc.im[i];
-    a[3 * i + 2] *= mc.im[i];
+    for (int i = 0; i < N; ++i) {
+      a[3 * i] *= pk.mc.im[i];
+      a[3 * i + 1] *= pk.mc.im[i];
+      a[3 * i + 2] *= pk.mc.im[i];
+    }
   }
   if constexpr (WITH_PE) pe = u;
 }
 
-template <class T, int N>
-__device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const MolConst<T, N>& mc) {
+template <class T, int N, bool PS>
+__device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const PairK<T, N, PS>& pk) {
   T u = T(0);
   static_for<0, N>([&](auto I) {
     constexpr int i = decltype(I)::value;
@@ -123,7 +154,7 @@ __device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const MolConst<T, N>& 
       const T ry = x[3 * i + 1] - x[3 * j + 1];
       const T rz = x[3 * i + 2] - x[3 * j + 2];
       const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
-      u = fmaT(mc.pc[pair_index<N>(i, j)], rsqrtT(d2), u);
+      u = fmaT(pk.template get<i, j>(), rsqrtT(d2), u);
     });
   });
   return u;
@@ -150,10 +181,18 @@ __device__ __forceinline__ T stage_sum(const T (&k)[7][M], int q, CoefFn coef) {
   return acc;
 }
 
+template <class CoefFn, int L>
+constexpr bool has_terms() {
+  bool any = false;
+  for (int l = 0; l < L; ++l) any = any || CoefFn::value_at(l) != 0.0;
+  return any;
+}
+
 template <int S>
 struct CoefAA {
   template <int L>
   static constexpr double value() { return kDP5.aa[S][L]; }
+  static constexpr double value_at(int l) { return kDP5.aa[S][l]; }
 };
 struct CoefB {
   template <int L>
@@ -170,9 +209,9 @@ struct CoefEA {
 
 // Hairer / scipy initial step (src/integrators.cpp:197-250) for one lane, in
 // the scalar reference's accumulation order; k0 = a(x0).
-template <class T, int N>
+template <class T, int N, bool PS>
 __device__ T lane_hairer(const T (&x)[3 * N], const T (&v)[3 * N], const T (&a0)[3 * N], T rtol,
-                         T atol, T max_dt, const MolConst<T, N>& mc) {
+                         T atol, T max_dt, const PairK<T, N, PS>& pk) {
   T d0s = T(0), d1s = T(0);
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) {
@@ -195,7 +234,7 @@ __device__ T lane_hairer(const T (&x)[3 * N], const T (&v)[3 * N], const T (&a0)
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) xs[q] = x[q] + v[q] * h0;
   T unused;
-  lane_force<T, N, false>(xs, a1, unused, mc);
+  lane_force<T, N, PS, false>(xs, a1, unused, pk);
   T d2s = T(0);
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) {
@@ -231,14 +270,29 @@ enum : int {
   kRunning = 255,
 };
 
+// One group of simulations of the same molecule size N inside a launch.
+struct Segment {
+  uint64_t start;     // first global sim index
+  uint64_t count;
+  uint64_t pos_off;   // offset (in elements) of sim `start` in geom / mom / mom32
+  uint64_t q_off;     // offset of sim `start` in qsim (per-sim charges)
+  int32_t n;          // atoms
+  int32_t per_sim_charges;
+};
+constexpr int kMaxSegments = 4;
+// ctr layout: [0] attempted steps, [1] accepted steps, [2] refill events,
+// [4 + s] work-queue head of segment s.
+constexpr int kCtrQueue = 4;
+constexpr int kCtrWords = 8;
+
 struct RunArgs {
-  const double* geom;         // [sim][N][3]
-  const uint8_t* geom_status; // nullptr, or non-zero = sampler failure
-  uint64_t n_sims;
-  unsigned long long* ctr;    // [0] queue head [1] attempts [2] accepted [3] refills
-  double* mom;                // [sim][N][3]
-  float* mom32;               // [sim][N][3]
-  uint32_t* steps;
+  const double* geom;         // [sim][n][3] per segment at pos_off
+  const uint8_t* geom_status; // nullptr, or non-zero = sampler failure (global sim index)
+  const float* qsim;          // per-sim charges [sim][n] per segment at q_off, or nullptr
+  unsigned long long* ctr;
+  double* mom;                // [sim][n][3] per segment at pos_off
+  float* mom32;
+  uint32_t* steps;            // global sim index
   uint32_t* attempts;
   uint8_t* status;
   double* t_final;
@@ -246,23 +300,34 @@ struct RunArgs {
   double* trace_h;            // [sim][trace_cap] or nullptr
   uint32_t trace_cap;
   int32_t stop_rule;
-  double rtol, atol, max_dt, t_max, pe_stop_fraction, momentum_tol, h0;
+  double rtol, atol, max_dt, t_max, pe_stop_fraction, momentum_tol, h0, coulomb_k;
   uint64_t max_steps;
   int32_t redistribute;
   int32_t h0_mode;
   int32_t max_rej;
   int32_t refill_threshold;   // 1..64
   int32_t chunk;              // >= 64
+  int32_t n_segments;
+  int32_t queue_slot;         // single-size launches: which queue head seg[0] uses
+  Segment seg[kMaxSegments];
 };
 
-// Persistent fused DP5(4): each lane integrates one simulation at a time with
-// its own step size; a wave pulls new simulation indices from a device work
-// queue (one atomic per `chunk` sims) whenever at least `refill_threshold` of
-// its lanes are idle, so converged lanes do not wait for stragglers.  Every
-// loop iteration is exactly one attempted step for every running lane, so the
-// wave stays convergent except for the short init / finalize branches.
-template <class T, int N>
-__global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
+struct Totals {
+  uint32_t att = 0, acc = 0, refills = 0;
+};
+
+// Runs one segment to exhaustion with a persistent per-lane DP5(4): each lane
+// integrates one simulation at a time with its own step size; the wave pulls
+// new simulation indices from the segment's device work queue (one atomic per
+// `chunk` sims) whenever at least `refill_threshold` of its lanes are idle, so
+// converged lanes do not wait for stragglers.  Every loop iteration is exactly
+// one attempted step for every running lane, so the wave stays convergent
+// except for the short init / finalize branches.  Returns when the queue is
+// exhausted and every lane of the wave has finished.
+template <class T, int N, bool PS>
+__device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
+                                            const MolConst<T, N>& mc,
+                                            unsigned long long* queue, Totals& tot) {
   constexpr int N3 = 3 * N;
   const T rtol = static_cast<T>(p.rtol), atol = static_cast<T>(p.atol);
   const T max_dt = static_cast<T>(p.max_dt);
@@ -270,16 +335,16 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
   const T tol2 = static_cast<T>(p.momentum_tol * p.momentum_tol);
   const int lane = __lane_id();
 
+  PairK<T, N, PS> pk{mc};
   T x[N3], v[N3], k[7][N3];
   T h = T(0);
   double t = 0.0, e0 = 0.0, pe_stop = 0.0;
   uint32_t steps = 0, att = 0;
   int nrej = 0;
   bool rej_once = false;
-  uint64_t sim = 0;
+  uint64_t loc = 0;  // index of the lane's sim inside the segment
   bool active = false;
   int status = kRunning;
-  uint32_t tot_att = 0, tot_acc = 0, refills = 0;
   uint64_t cbase = 0, cend = 0;
   bool exhausted = false;
 
@@ -288,21 +353,21 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
     const uint64_t idle = __ballot(!active);
     const int n_idle = __popcll(idle);
     if (!exhausted && n_idle >= p.refill_threshold) {
-      ++refills;
+      ++tot.refills;
       const int rank = static_cast<int>(lanes_below(idle));
       int assigned = 0;
       uint64_t mine = ~0ull;
       while (assigned < n_idle) {
         if (cbase >= cend) {
           uint64_t b = 0;
-          if (lane == 0) b = atomicAdd(p.ctr, static_cast<unsigned long long>(p.chunk));
+          if (lane == 0) b = atomicAdd(queue, static_cast<unsigned long long>(p.chunk));
           b = bcast64(b);
-          if (b >= p.n_sims) {
+          if (b >= sg.count) {
             exhausted = true;
             break;
           }
           cbase = b;
-          cend = (b + p.chunk < p.n_sims) ? b + p.chunk : p.n_sims;
+          cend = (b + p.chunk < sg.count) ? b + p.chunk : sg.count;
         }
         const uint64_t left = cend - cbase;
         const int take = (static_cast<uint64_t>(n_idle - assigned) < left)
@@ -314,7 +379,7 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
         assigned += take;
       }
       if (!active && mine != ~0ull) {
-        sim = mine;
+        loc = mine;
         active = true;
         steps = 0;
         att = 0;
@@ -322,18 +387,27 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
         rej_once = false;
         t = 0.0;
         status = kRunning;
-        const double* g = p.geom + sim * N3;
+        const double* g = p.geom + sg.pos_off + loc * N3;
 #pragma unroll
         for (int q = 0; q < N3; ++q) {
           x[q] = static_cast<T>(g[q]);
           v[q] = T(0);
         }
-        if (p.geom_status != nullptr && p.geom_status[sim] != 0) {
+        if constexpr (PS) {
+          const float* qs = p.qsim + sg.q_off + loc * N;
+#pragma unroll
+          for (int i = 0; i < N; ++i) {
+            pk.q[i] = static_cast<T>(qs[i]);
+            pk.kq[i] = static_cast<T>(p.coulomb_k * static_cast<double>(qs[i]));
+          }
+        }
+        if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
           status = kSimSamplerFail;
           e0 = 0.0;
         } else {
           T pe0;
-          lane_force<T, N, true>(x, k[0], pe0, mc);
+          // The stages hold forces F = m a; 1/m enters via per-atom step factors.
+          lane_force<T, N, PS, true, false>(x, k[0], pe0, pk);
           e0 = static_cast<double>(pe0);
           pe_stop = p.pe_stop_fraction * e0;
           if (p.stop_rule == kStopPE && e0 <= pe_stop) {
@@ -341,8 +415,14 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
           } else if (p.max_steps == 0) {
             status = kSimNotConverged;
           } else {
-            h = (p.h0_mode == kH0Fixed) ? static_cast<T>(p.h0)
-                                        : lane_hairer<T, N>(x, v, k[0], rtol, atol, max_dt, mc);
+            if (p.h0_mode == kH0Fixed) {
+              h = static_cast<T>(p.h0);
+            } else {
+              T a0[N3];
+#pragma unroll
+              for (int q = 0; q < N3; ++q) a0[q] = k[0][q] * mc.im[q / 3];
+              h = lane_hairer<T, N, PS>(x, v, a0, rtol, atol, max_dt, pk);
+            }
           }
         }
       }
@@ -361,34 +441,46 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
         }
       }
       const T hh = minT(h, cap);
+      const T hh2 = hh * hh;
+      T g[N], gv[N];  // h^2 / m_i and h / m_i
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        g[i] = hh2 * mc.im[i];
+        gv[i] = hh * mc.im[i];
+      }
       ++att;
       T xs[N3], vn[N3];
-      // Stages 2..6: positions x + h (c_s v + h sum_l aa[s][l] k_l), then a(x_s).
+      // Stages 2..6: positions x + h c_s v + (h^2/m) sum_l aa[s][l] F_l, then F(x_s).
       static_for<1, 6>([&](auto Sc) {
         constexpr int s = decltype(Sc)::value;
-        constexpr T cs = static_cast<T>(kDP5.c[s]);
+        const T hc = hh * static_cast<T>(kDP5.c[s]);
 #pragma unroll
         for (int q = 0; q < N3; ++q) {
-          const T acc = stage_sum<T, s>(k, q, CoefAA<s>{});
-          xs[q] = fmaT(hh, fmaT(cs, v[q], hh * acc), x[q]);
+          const T base = fmaT(hc, v[q], x[q]);
+          if constexpr (has_terms<CoefAA<s>, s>()) {
+            const T acc = stage_sum<T, s>(k, q, CoefAA<s>{});
+            xs[q] = fmaT(g[q / 3], acc, base);
+          } else {
+            xs[q] = base;
+          }
         }
         T unused;
-        lane_force<T, N, false>(xs, k[s], unused, mc);
+        lane_force<T, N, PS, false, false>(xs, k[s], unused, pk);
       });
-      // Stage 7 (FSAL): the 5th-order solution; a(x_new) is the next k1.
+      // Stage 7 (FSAL): the 5th-order solution; F(x_new) is the next k1.
 #pragma unroll
       for (int q = 0; q < N3; ++q) {
         const T ax = stage_sum<T, 6>(k, q, CoefAA<6>{});
         const T av = stage_sum<T, 6>(k, q, CoefB{});
-        xs[q] = fmaT(hh, fmaT(hh, ax, v[q]), x[q]);
-        vn[q] = fmaT(hh, av, v[q]);
+        xs[q] = fmaT(g[q / 3], ax, fmaT(hh, v[q], x[q]));
+        vn[q] = fmaT(gv[q / 3], av, v[q]);
       }
       {
         T unused;
-        lane_force<T, N, false>(xs, k[6], unused, mc);
+        lane_force<T, N, PS, false, false>(xs, k[6], unused, pk);
       }
-      // Embedded error in the RMS norm, scale = atol + rtol max(|y_old|, |y_new|).
-      const T hh2 = hh * hh;
+      // Embedded error in the RMS norm, scale = atol + rtol max(|y_old|, |y_new|);
+      // the squared norm is compared with 1.
       T exs = T(0), evs = T(0);
 #pragma unroll
       for (int q = 0; q < N3; ++q) {
@@ -396,13 +488,15 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
         const T ev = stage_sum<T, 7>(k, q, CoefE{});
         const T scx = fmaT(rtol, maxT(absT(x[q]), absT(xs[q])), atol);
         const T scv = fmaT(rtol, maxT(absT(v[q]), absT(vn[q])), atol);
-        const T rx = (hh2 * ex) * rcpT(scx);
-        const T rv = (hh * ev) * rcpT(scv);
+        const T rx = (g[q / 3] * ex) * rcpT(scx);
+        const T rv = (gv[q / 3] * ev) * rcpT(scv);
         exs = fmaT(rx, rx, exs);
         evs = fmaT(rv, rv, evs);
       }
-      const T en = sqrtT((exs + evs) * inv6n);
-      if (en <= T(1)) {
+      const T en2 = (exs + evs) * inv6n;
+      // 0.9 err^(-1/5) with err = sqrt(en2): 0.9 * 2^(-0.1 log2 en2).
+      const T grow = T(0.9) * exp2T(T(-0.1) * log2T(en2));
+      if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
 #pragma unroll
@@ -421,19 +515,19 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
           v[q] = vn[q];
           k[0][q] = k[6][q];
         }
-        T f = (en == T(0)) ? T(10) : minT(T(10), T(0.9) * powT(en, T(-0.2)));
+        T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
         h = minT(hh * f, cap);
         rej_once = false;
         nrej = 0;
         if (p.trace_h != nullptr && steps < p.trace_cap)
-          p.trace_h[sim * p.trace_cap + steps] = static_cast<double>(hh);
+          p.trace_h[(sg.start + loc) * p.trace_cap + steps] = static_cast<double>(hh);
         ++steps;
-        ++tot_acc;
+        ++tot.acc;
         const bool final_step = by_time && hh >= cap;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
-          if (static_cast<double>(lane_pe<T, N>(x, mc)) <= pe_stop) status = kSimOkPE;
+          if (static_cast<double>(lane_pe<T, N, PS>(x, pk)) <= pe_stop) status = kSimOkPE;
         } else if (settled) {
           status = kSimOkMomentum;
         } else if (final_step) {
@@ -441,10 +535,10 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
         }
         if (status == kRunning && steps >= p.max_steps) status = kSimNotConverged;
       } else {
-        const T f = maxT(T(0.2), T(0.9) * powT(en, T(-0.2)));
+        const T f = maxT(T(0.2), grow);
         h = hh * f;
         rej_once = true;
-        if (++nrej > p.max_rej) status = (en != en) ? kSimNonfinite : kSimRejectLimit;
+        if (++nrej > p.max_rej) status = (en2 != en2) ? kSimNonfinite : kSimRejectLimit;
       }
     }
 
@@ -465,42 +559,76 @@ __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const Mol
           const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
           const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
           const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
-          pe_end += mc.pcd[pair_index<N>(i, j)] / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+          pe_end += pk.template getd<i, j>() / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
         });
       });
+      const uint64_t base = sg.pos_off + loc * N3;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         const double ms = mc.md[i] * s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double pm = ms * static_cast<double>(v[3 * i + c]);
-          p.mom[sim * N3 + 3 * i + c] = pm;
-          p.mom32[sim * N3 + 3 * i + c] = static_cast<float>(pm);
+          p.mom[base + 3 * i + c] = pm;
+          p.mom32[base + 3 * i + c] = static_cast<float>(pm);
         }
       }
-      p.steps[sim] = steps;
-      p.attempts[sim] = att;
-      p.status[sim] = static_cast<uint8_t>(status);
-      p.t_final[sim] = t;
-      p.eerr[sim] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+      const uint64_t gs = sg.start + loc;
+      p.steps[gs] = steps;
+      p.attempts[gs] = att;
+      p.status[gs] = static_cast<uint8_t>(status);
+      p.t_final[gs] = t;
+      p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
       if (p.trace_h != nullptr)
-        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[sim * p.trace_cap + q] = 0.0;
-      tot_att += att;
+        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
+      tot.att += att;
       active = false;
     }
   }
+}
 
-  // ---- device totals: one atomic per wave ----
+__device__ __forceinline__ void flush_totals(const RunArgs& p, Totals& tot) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
-    tot_att += __shfl_xor(tot_att, off);
-    tot_acc += __shfl_xor(tot_acc, off);
+    tot.att += __shfl_xor(tot.att, off);
+    tot.acc += __shfl_xor(tot.acc, off);
   }
-  if (lane == 0) {
-    atomicAdd(p.ctr + 1, static_cast<unsigned long long>(tot_att));
-    atomicAdd(p.ctr + 2, static_cast<unsigned long long>(tot_acc));
-    atomicAdd(p.ctr + 3, static_cast<unsigned long long>(refills));
+  if (__lane_id() == 0) {
+    atomicAdd(p.ctr + 0, static_cast<unsigned long long>(tot.att));
+    atomicAdd(p.ctr + 1, static_cast<unsigned long long>(tot.acc));
+    atomicAdd(p.ctr + 2, static_cast<unsigned long long>(tot.refills));
   }
+}
+
+// One molecule for the whole launch (segment 0).
+template <class T, int N, bool PS>
+__global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
+  Totals tot;
+  run_segment<T, N, PS>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
+  flush_totals(p, tot);
+}
+
+// Several molecule sizes in one launch: the segments are taken in order, each
+// wave moving to the next segment as soon as the current one's queue is empty
+// and its own lanes have finished (no grid-wide barrier between segments).  The
+// sizes are compile-time (N0..N3); segment s must have size N_s.
+template <class T, int N0, int N1, int N2, int N3>
+struct MixedConst {
+  MolConst<T, N0> m0;
+  MolConst<T, N1> m1;
+  MolConst<T, N2> m2;
+  MolConst<T, N3> m3;
+};
+
+template <class T, int N0, int N1, int N2, int N3, bool PS>
+__global__ void __launch_bounds__(256) dp5_mixed(const RunArgs p,
+                                                 const MixedConst<T, N0, N1, N2, N3> mc) {
+  Totals tot;
+  if (p.n_segments > 0) run_segment<T, N0, PS>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
+  if (p.n_segments > 1) run_segment<T, N1, PS>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);
+  if (p.n_segments > 2) run_segment<T, N2, PS>(p, p.seg[2], mc.m2, p.ctr + kCtrQueue + 2, tot);
+  if (p.n_segments > 3) run_segment<T, N3, PS>(p, p.seg[3], mc.m3, p.ctr + kCtrQueue + 3, tot);
+  flush_totals(p, tot);
 }
 
 // ---------------------------------------------------------------- sampler ----
@@ -524,72 +652,105 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 }
 
 struct SampleArgs {
-  double* geom;        // [sim][n][3]
-  uint8_t* status;     // 0 ok / kSimSamplerFail
-  uint64_t first_index;
+  double* geom;        // [sim][n][3], this segment's first sim
+  uint8_t* status;     // 0 ok / kSimSamplerFail, this segment's first sim
+  float* charges;      // [sim][n] per-sim charges or nullptr
+  uint64_t first_index;  // global index of the segment's first sim
   uint64_t n_sims;
   uint32_t key0, key1;
   float R, twoR, R2, min2;
   uint32_t max_attempts;
   int32_t n_atoms;
+  int32_t charge_levels;   // per-sim charges uniform in {1..levels}
+  int32_t max_total_charge;  // 0 = no limit
 };
 
 // One thread per simulation.  Candidate j of global sim g draws Philox4x32-10 on
 // counter (j, lo32(g), hi32(g), 0): 24-bit uniforms -> fma(u, 2R, -R); kept if
 // inside the ball and >= min_sep from every atom already placed
-// (src/sampler.cpp:28-57 placement rule).
+// (src/sampler.cpp:28-57 placement rule).  Per-sim charges, when asked for, use
+// counter (4t + i/4, lo32(g), hi32(g), 1), word i%4, q_i = 1 + (u24 * L >> 24),
+// redrawn (t+1) while their sum exceeds max_total_charge.
 template <int N>
 __global__ void __launch_bounds__(256) sample_sphere(const SampleArgs a) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
   for (uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; s < a.n_sims;
        s += stride) {
     const uint64_t g = a.first_index + s;
-    uint32_t cand = 0;
+    // One candidate per iteration, so a wave iterates until its slowest lane
+    // has placed every atom (not atom by atom).  The draw sequence is the same
+    // as placing atom by atom: candidate j, cube rejection (at most 4096 draws
+    // per point), then the separation test against the atoms already placed.
+    uint32_t cand = 0, cube = 0, tries = 0;
+    int placed = 0;
     float P[3 * N];
     bool ok = true;
+    while (ok && placed < N) {
+      uint32_t r[4] = {cand++, static_cast<uint32_t>(g), static_cast<uint32_t>(g >> 32), 0u};
+      philox4x32_10(r, a.key0, a.key1);
+      float c[3];
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      if (!ok) break;
-      uint32_t tries = 0;
-      for (;;) {
-        float c[3];
-        bool in_ball = false;
-        for (int cube = 0; cube < 4096; ++cube) {
-          uint32_t r[4] = {cand++, static_cast<uint32_t>(g), static_cast<uint32_t>(g >> 32), 0u};
-          philox4x32_10(r, a.key0, a.key1);
+      for (int q = 0; q < 3; ++q) {
+        const float u = static_cast<float>(r[q] >> 8) * 0x1.0p-24f;
+        c[q] = __builtin_fmaf(u, a.twoR, -a.R);
+      }
+      const float d = __builtin_fmaf(c[2], c[2], __builtin_fmaf(c[1], c[1], c[0] * c[0]));
+      if (!(d <= a.R2)) {
+        ok = ++cube < 4096;
+        continue;
+      }
+      cube = 0;
+      bool clears = true;
 #pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            const float u = static_cast<float>(r[q] >> 8) * 0x1.0p-24f;
-            c[q] = __builtin_fmaf(u, a.twoR, -a.R);
-          }
-          const float d = __builtin_fmaf(c[2], c[2], __builtin_fmaf(c[1], c[1], c[0] * c[0]));
-          if (d <= a.R2) {
-            in_ball = true;
-            break;
-          }
-        }
-        if (!in_ball) {
-          ok = false;
-          break;
-        }
-        bool clears = true;
-#pragma unroll
-        for (int j = 0; j < i; ++j) {
+      for (int j = 0; j < N; ++j) {
+        if (j < placed) {
           const float dx = c[0] - P[3 * j], dy = c[1] - P[3 * j + 1], dz = c[2] - P[3 * j + 2];
           const float d2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
           clears = clears && !(d2 < a.min2);
         }
-        if (clears) {
-          P[3 * i] = c[0];
-          P[3 * i + 1] = c[1];
-          P[3 * i + 2] = c[2];
-          break;
-        }
-        if (++tries >= a.max_attempts) {
-          ok = false;
-          break;
-        }
       }
+      if (clears) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          if (j == placed) {
+            P[3 * j] = c[0];
+            P[3 * j + 1] = c[1];
+            P[3 * j + 2] = c[2];
+          }
+        }
+        ++placed;
+        tries = 0;
+      } else {
+        ok = ++tries < a.max_attempts;
+      }
+    }
+    if (a.charges != nullptr) {
+      float Q[N];
+      bool placed = false;
+      for (uint32_t t = 0; t < 4096 && !placed; ++t) {
+        int total = 0;
+#pragma unroll
+        for (int b = 0; b < (N + 3) / 4; ++b) {
+          uint32_t r[4] = {4u * t + static_cast<uint32_t>(b), static_cast<uint32_t>(g),
+                           static_cast<uint32_t>(g >> 32), 1u};
+          philox4x32_10(r, a.key0, a.key1);
+#pragma unroll
+          for (int w = 0; w < 4; ++w) {
+            const int i = 4 * b + w;
+            if (i < N) {
+              const int qi = 1 + static_cast<int>((static_cast<uint64_t>(r[w] >> 8) *
+                                                   static_cast<uint32_t>(a.charge_levels)) >> 24);
+              Q[i] = static_cast<float>(qi);
+              total += qi;
+            }
+          }
+        }
+        placed = a.max_total_charge <= 0 || total <= a.max_total_charge;
+      }
+      ok = ok && placed;
+      float* qo = a.charges + s * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) qo[i] = ok ? Q[i] : 0.0f;
     }
     double* out = a.geom + s * (3 * N);
 #pragma unroll
@@ -705,7 +866,7 @@ __global__ void __launch_bounds__(kRecBlock) pack_records(const double* geom, co
   for (uint64_t i = n16 * 16 + threadIdx.x; i < bytes; i += kRecBlock) dst[i] = stage[i];
 }
 
-__global__ void zero_u64(unsigned long long* p, int n) {
+static __global__ void zero_u64(unsigned long long* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }

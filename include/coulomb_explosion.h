@@ -128,8 +128,27 @@ typedef struct {
   int32_t grid_blocks;
   int32_t block_threads;
   int32_t n_cus;
-  int32_t pad_;
+  int32_t launches;          /* integration kernels launched (1 unless split by size) */
 } ce_run_stats;
+
+/* One molecule type of a mixed-size engine (no reference counterpart: the
+ * reference runs one fixed molecule per batch, batched_force.hpp:9-10).  Its
+ * `count` sims occupy consecutive global indices after the previous species'.
+ * charge_levels = 0 uses `charges` for every sim; L > 0 gives each sim its own
+ * charges, drawn uniformly from {1..L} by the device sampler (redrawn while
+ * their sum exceeds max_total_charge, if that is > 0). */
+#define CE_MAX_SPECIES 4
+typedef struct {
+  int32_t n_atoms;
+  int32_t charge_levels;
+  const double* masses;   /* n_atoms, electron masses */
+  const double* charges;  /* n_atoms (may be NULL when charge_levels > 0) */
+  double radius;          /* device-sampler ball radius, bohr */
+  double min_separation;
+  uint64_t count;
+  int32_t max_total_charge;
+  int32_t pad_;
+} ce_species_desc;
 
 /* Launch shape: 0 fields mean "engine default".  refill_threshold = idle lanes a
  * wave waits for before it pulls new sims (1..64); chunk = sims a wave claims
@@ -153,6 +172,22 @@ void ce_run_config_default(ce_run_config* cfg);
 int ce_engine_create(const ce_molecule_desc* mol, const ce_tolerances* tol, size_t capacity,
                      ce_engine** out);
 int ce_engine_destroy(ce_engine* eng);
+/* A mixed-size engine (1..CE_MAX_SPECIES species) run in one launch: the
+ * species occupy global sims species-major; geometry and momenta arrays are
+ * species-major with each species' own atom count.  Sizes (2, 3, 5, 8) in that
+ * order run as one persistent kernel; other combinations run one kernel per
+ * species on the engine stream. */
+int ce_engine_create_mixed(const ce_species_desc* species, int32_t n_species, double coulomb_k,
+                           const ce_tolerances* tol, ce_engine** out);
+/* Device sampler for every species of a mixed engine: geometry from each
+ * species' radius / min_separation, and per-sim charges where charge_levels > 0;
+ * sim g of the engine has global index first_index + g. */
+int ce_engine_sample_species(ce_engine* eng, uint64_t seed, uint64_t first_index,
+                             uint32_t max_attempts);
+/* Per-sim charges for the loaded sims ([sim][atom], species-major like the
+ * geometry), replacing the molecule's fixed charges on the next runs. */
+int ce_engine_set_charges(ce_engine* eng, const double* charges, size_t n_sims);
+int ce_engine_get_charges(ce_engine* eng, double* charges, size_t n_sims);
 
 /* Replaces set_lane_geometry(lane, positions) (batched_integrator.hpp:112-121)
  * for sims [0, n_sims): pos is AoS doubles [(sim*n + i)*3 + c], velocities start

@@ -124,6 +124,12 @@ int ceo_lane_run_f32(const ceo_molecule* mol, const ceo_config* cfg, const doubl
                      size_t n_sims, ceo_results* out);
 int ceo_lane_run_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                      size_t n_sims, ceo_results* out);
+/* The same with per-sim charges charges[sim][atom] (the kernel's per-sim-charge
+ * path: kq_i = (T)(k q_i), pair constant kq_i * q_j). */
+int ceo_lane_run_q_f32(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                       const double* charges, size_t n_sims, ceo_results* out);
+int ceo_lane_run_q_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                       const double* charges, size_t n_sims, ceo_results* out);
 /* The derived coefficient tables of the per-lane form, in double:
  * c[7], aa[7*7] (A*A), b[7], e[7], ea[7] (E*A).  Written for inspection. */
 void ceo_lane_coefficients(double* c, double* aa, double* b, double* e, double* ea);
@@ -134,6 +140,13 @@ void ceo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 size_t ceo_device_sample(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
                          double radius, double min_sep, uint32_t max_attempts, double* pos_out,
                          uint8_t* status_out);
+
+/* With per-sim charges uniform in {1..charge_levels} (q_out [sim][atom]), sums
+ * capped by max_total (> 0). */
+size_t ceo_device_sample_q(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
+                           double radius, double min_sep, uint32_t max_attempts,
+                           int32_t charge_levels, int32_t max_total, double* pos_out,
+                           double* q_out, uint8_t* status_out);
 
 /* ---- CPU baseline: lockstep f32 at the host SIMD width, n_threads instances.
  * Runs `reps` timed passes over the sims and reports the median wall time.

@@ -188,6 +188,19 @@ class Oracle:
                                            max_attempts, out.ctypes.data, st.ctypes.data)
         return out, st, int(fails)
 
+    def device_sample_q(self, seed, first_index, n_sims, n_atoms, radius, min_sep, levels,
+                        max_total=0, max_attempts=10000):
+        f = self.lib.ceo_device_sample_q
+        f.argtypes = [C.c_uint64, C.c_uint64, C.c_size_t, C.c_int32, C.c_double, C.c_double,
+                      C.c_uint32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
+        f.restype = C.c_size_t
+        out = np.zeros((n_sims, n_atoms, 3), dtype=np.float64)
+        q = np.zeros((n_sims, n_atoms), dtype=np.float64)
+        st = np.zeros(n_sims, dtype=np.uint8)
+        fails = f(seed, first_index, n_sims, n_atoms, radius, min_sep, max_attempts, levels,
+                  max_total, out.ctypes.data, q.ctypes.data, st.ctypes.data)
+        return out, q, st, int(fails)
+
     def philox(self, ctr, key):
         c = np.ascontiguousarray(ctr, dtype=np.uint32)
         k = np.ascontiguousarray(key, dtype=np.uint32)
@@ -233,6 +246,16 @@ class Oracle:
     def lane_run(self, masses, charges, pos, cfg, k=1.0, precision="f32", trace_cap=0):
         fn = self.lib.ceo_lane_run_f32 if precision == "f32" else self.lib.ceo_lane_run_f64
         return self._run(fn, masses, charges, pos, cfg, k, trace_cap=trace_cap)
+
+    def lane_run_q(self, masses, charges_per_sim, pos, cfg, k=1.0, precision="f32", trace_cap=0):
+        """Per-lane form with per-sim charges [sims, n]."""
+        q = np.ascontiguousarray(charges_per_sim, dtype=np.float64)
+        fn = self.lib.ceo_lane_run_q_f32 if precision == "f32" else self.lib.ceo_lane_run_q_f64
+        fn.argtypes = [C.c_void_p] * 4 + [C.c_size_t, C.c_void_p]
+
+        def call(mol, cfg_, pos_, ns, res):
+            return fn(mol, cfg_, pos_, q.ctypes.data, ns, res)
+        return self._run(call, masses, np.ones(len(masses)), pos, cfg, k, trace_cap=trace_cap)
 
     def batched_run(self, masses, charges, pos, cfg, lanes=8, k=1.0, precision="f32"):
         fn = self.lib.ceo_batched_run_f32 if precision == "f32" else self.lib.ceo_batched_run_f64

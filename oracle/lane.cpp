@@ -40,18 +40,25 @@ const LaneTableD& lane_table() {
     LaneTableD r{};
     const double c[7] = {0.0, 1.0 / 5, 3.0 / 10, 4.0 / 5, 8.0 / 9, 1.0, 1.0};
     for (int s = 0; s < 7; ++s) r.c[s] = c[s];
+    // (A A)[s][l] = sum_{l<j<s} a[s][j] a[j][l] and (E A)[l] = sum_{s>l} e[s] a[s][l],
+    // as exact rationals (tests/test_oracle.py recomputes them from the tableau in
+    // rational arithmetic).
+    const double aa[7][7] = {
+        {0, 0, 0, 0, 0, 0, 0},
+        {0, 0, 0, 0, 0, 0, 0},
+        {9.0 / 200, 0, 0, 0, 0, 0, 0},
+        {-12.0 / 25, 4.0 / 5, 0, 0, 0, 0, 0},
+        {-12248.0 / 6561, 7208.0 / 2187, -6784.0 / 6561, 0, 0, 0, 0},
+        {-533.0 / 264, 91.0 / 22, -56.0 / 33, 7.0 / 88, 0, 0, 0},
+        {35.0 / 384, 0, 50.0 / 159, 25.0 / 192, -243.0 / 6784, 0, 0}};
+    const double ea[7] = {611.0 / 230400,     0, -514.0 / 83475, 391.0 / 38400,
+                          -4617.0 / 1356800, -11.0 / 3360, 0};
     for (int s = 0; s < 7; ++s)
-      for (int l = 0; l < 7; ++l) {
-        double acc = 0.0;
-        for (int j = l + 1; j < s; ++j) acc += d.a[s][j] * d.a[j][l];
-        r.aa[s][l] = acc;
-      }
+      for (int l = 0; l < 7; ++l) r.aa[s][l] = aa[s][l];
     for (int l = 0; l < 7; ++l) {
       r.b[l] = d.a[6][l];
       r.e[l] = d.e[l];
-      double acc = 0.0;
-      for (int s = l + 1; s < 7; ++s) acc += d.e[s] * d.a[s][l];
-      r.ea[l] = acc;
+      r.ea[l] = ea[l];
     }
     return r;
   }();
@@ -91,6 +98,20 @@ struct LaneMol {
   double md[kMaxN];
 };
 
+// Per-sim charges, as the kernel builds them: kq_i = (T)(k q_i), pc_ij = kq_i * (T)q_j,
+// and the FP64 diagnostic constant (double)kq_i * (double)q_j.
+template <class T>
+void set_lane_charges(LaneMol<T>& r, const ceo_molecule* m, const double* q) {
+  for (int i = 0; i < m->n; ++i) {
+    const T kq = static_cast<T>(m->coulomb_k * static_cast<double>(static_cast<float>(q[i])));
+    for (int j = 0; j < m->n; ++j) {
+      const T qj = static_cast<T>(static_cast<float>(q[j]));
+      r.pc[i][j] = kq * qj;
+      r.pcd[i][j] = static_cast<double>(kq) * static_cast<double>(qj);
+    }
+  }
+}
+
 template <class T>
 LaneMol<T> lane_mol(const ceo_molecule* m) {
   LaneMol<T> r{};
@@ -108,7 +129,7 @@ LaneMol<T> lane_mol(const ceo_molecule* m) {
 
 // Accelerations (and optionally PE) of one lane: x, a are [3n] (atom-major).
 template <class T>
-void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe) {
+void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe, bool apply_im = true) {
   const int n = m.n;
   for (int q = 0; q < 3 * n; ++q) a[q] = T(0);
   T u = T(0);
@@ -130,11 +151,12 @@ void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe) {
       a[3 * j + 1] -= fy;
       a[3 * j + 2] -= fz;
     }
-  for (int i = 0; i < n; ++i) {
-    a[3 * i] *= m.im[i];
-    a[3 * i + 1] *= m.im[i];
-    a[3 * i + 2] *= m.im[i];
-  }
+  if (apply_im)
+    for (int i = 0; i < n; ++i) {
+      a[3 * i] *= m.im[i];
+      a[3 * i + 1] *= m.im[i];
+      a[3 * i + 2] *= m.im[i];
+    }
   if (pe) *pe = u;
 }
 
@@ -206,7 +228,8 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
     v[q] = T(0);
   }
   T pe0;
-  lane_force(m, x, k[0], &pe0);
+  // The stages hold forces F = m a; 1/m enters through per-atom step factors.
+  lane_force(m, x, k[0], &pe0, false);
   const double e0 = static_cast<double>(pe0);
   const double pe_stop = cfg->pe_stop_fraction * e0;
   const T tol2 = static_cast<T>(cfg->momentum_tol * cfg->momentum_tol);
@@ -223,9 +246,12 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
   } else if (cfg->max_steps == 0) {
     status = CEO_NOT_CONVERGED;
     h = T(0);
+  } else if (cfg->h0_mode == CEO_H0_FIXED) {
+    h = static_cast<T>(cfg->h0);
   } else {
-    h = (cfg->h0_mode == CEO_H0_FIXED) ? static_cast<T>(cfg->h0)
-                                       : lane_hairer(m, x, v, k[0], rtol, atol, max_dt);
+    T a0[3 * kMaxN];
+    for (int q = 0; q < n3; ++q) a0[q] = k[0][q] * m.im[q / 3];
+    h = lane_hairer(m, x, v, a0, rtol, atol, max_dt);
   }
   const T inv6n = T(1) / static_cast<T>(6 * n);
   while (status == 255) {
@@ -239,9 +265,16 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
     }
     const T hh = std::min(h, cap);
+    const T hh2 = hh * hh;
+    T g[kMaxN], gv[kMaxN];  // h^2 / m_i and h / m_i
+    for (int i = 0; i < n; ++i) {
+      g[i] = hh2 * m.im[i];
+      gv[i] = hh * m.im[i];
+    }
     ++attempts;
-    // Stages 2..6: positions only.
+    // Stages 2..6: positions x + h c_s v + (h^2/m) sum_l aa[s][l] F_l, then F(x_s).
     for (int s = 1; s < 6; ++s) {
+      const T hc = hh * K.c[s];
       for (int q = 0; q < n3; ++q) {
         T acc = T(0);
         bool first = true;
@@ -250,9 +283,10 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
           acc = first ? K.aa[s][l] * k[l][q] : std::fma(K.aa[s][l], k[l][q], acc);
           first = false;
         }
-        xn[q] = std::fma(hh, std::fma(K.c[s], v[q], hh * acc), x[q]);
+        const T base = std::fma(hc, v[q], x[q]);
+        xn[q] = first ? base : std::fma(g[q / 3], acc, base);
       }
-      lane_force(m, xn, k[s], static_cast<T*>(nullptr));
+      lane_force(m, xn, k[s], static_cast<T*>(nullptr), false);
     }
     // Stage 7 (FSAL): the 5th-order solution itself.
     for (int q = 0; q < n3; ++q) {
@@ -268,12 +302,11 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
           fv = false;
         }
       }
-      xn[q] = std::fma(hh, std::fma(hh, ax, v[q]), x[q]);
-      vn[q] = std::fma(hh, av, v[q]);
+      xn[q] = std::fma(g[q / 3], ax, std::fma(hh, v[q], x[q]));
+      vn[q] = std::fma(gv[q / 3], av, v[q]);
     }
-    lane_force(m, xn, k[6], static_cast<T*>(nullptr));
-    // Embedded error, RMS over the 6n components.
-    const T hh2 = hh * hh;
+    lane_force(m, xn, k[6], static_cast<T*>(nullptr), false);
+    // Embedded error, RMS over the 6n components (squared norm compared with 1).
     T ex_sum = T(0), ev_sum = T(0);
     for (int q = 0; q < n3; ++q) {
       T ex = T(0), ev = T(0);
@@ -290,13 +323,15 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
       const T scx = std::fma(rtol, std::max(std::abs(x[q]), std::abs(xn[q])), atol);
       const T scv = std::fma(rtol, std::max(std::abs(v[q]), std::abs(vn[q])), atol);
-      const T rx = (hh2 * ex) * (T(1) / scx);
-      const T rv = (hh * ev) * (T(1) / scv);
+      const T rx = (g[q / 3] * ex) * (T(1) / scx);
+      const T rv = (gv[q / 3] * ev) * (T(1) / scv);
       ex_sum = std::fma(rx, rx, ex_sum);
       ev_sum = std::fma(rv, rv, ev_sum);
     }
-    const T en = std::sqrt((ex_sum + ev_sum) * inv6n);
-    if (en <= T(1)) {
+    const T en2 = (ex_sum + ev_sum) * inv6n;
+    // 0.9 err^(-1/5) with err = sqrt(en2): 0.9 * 2^(-0.1 log2 en2).
+    const T grow = T(0.9) * std::exp2(T(-0.1) * std::log2(en2));
+    if (en2 <= T(1)) {
       bool settled = true;
       if (cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX) {
         for (int i = 0; i < n && settled; ++i) {
@@ -313,7 +348,7 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
         v[q] = vn[q];
         k[0][q] = k[6][q];
       }
-      T f = (en == T(0)) ? T(10) : std::min(T(10), T(0.9) * std::pow(en, T(-0.2)));
+      T f = (en2 == T(0)) ? T(10) : std::min(T(10), grow);
       if (rej_once) f = std::min(f, T(1));
       h = std::min(hh * f, cap);
       rej_once = false;
@@ -331,10 +366,10 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
       if (status == 255 && steps >= cfg->max_steps) status = CEO_NOT_CONVERGED;
     } else {
-      const T f = std::max(T(0.2), T(0.9) * std::pow(en, T(-0.2)));
+      const T f = std::max(T(0.2), grow);
       h = hh * f;
       rej_once = true;
-      if (++nrej > cfg->max_rejections) status = std::isnan(en) ? CEO_NONFINITE : CEO_REJECT_LIMIT;
+      if (++nrej > cfg->max_rejections) status = std::isnan(en2) ? CEO_NONFINITE : CEO_REJECT_LIMIT;
     }
   }
   // Finalize in double: redistribution s = sqrt(E0/KE), p = m s v.
@@ -372,11 +407,13 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
 
 template <class T>
 int lane_run(const ceo_molecule* cm, const ceo_config* cfg, const double* pos, size_t n_sims,
-             ceo_results* out) {
+             ceo_results* out, const double* charges) {
   if (cm->n < 1 || cm->n > kMaxN) return -1;
-  const LaneMol<T> m = lane_mol<T>(cm);
-  for (size_t sim = 0; sim < n_sims; ++sim)
+  LaneMol<T> m = lane_mol<T>(cm);
+  for (size_t sim = 0; sim < n_sims; ++sim) {
+    if (charges) set_lane_charges<T>(m, cm, charges + sim * cm->n);
     lane_run_one<T>(m, cfg, pos + sim * cm->n * 3, sim, out);
+  }
   return 0;
 }
 
@@ -402,12 +439,22 @@ extern "C" {
 
 int ceo_lane_run_f32(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                      size_t n_sims, ceo_results* out) {
-  return lane_run<float>(mol, cfg, pos, n_sims, out);
+  return lane_run<float>(mol, cfg, pos, n_sims, out, nullptr);
 }
 
 int ceo_lane_run_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                      size_t n_sims, ceo_results* out) {
-  return lane_run<double>(mol, cfg, pos, n_sims, out);
+  return lane_run<double>(mol, cfg, pos, n_sims, out, nullptr);
+}
+
+int ceo_lane_run_q_f32(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                       const double* charges, size_t n_sims, ceo_results* out) {
+  return lane_run<float>(mol, cfg, pos, n_sims, out, charges);
+}
+
+int ceo_lane_run_q_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                       const double* charges, size_t n_sims, ceo_results* out) {
+  return lane_run<double>(mol, cfg, pos, n_sims, out, charges);
 }
 
 void ceo_lane_coefficients(double* c, double* aa, double* b, double* e, double* ea) {
@@ -441,9 +488,26 @@ void ceo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 // (cube rejection) and clears every atom placed before it by min_sep
 // (src/sampler.cpp:28-57); an atom failing max_attempts separation tests, or a
 // candidate needing more than 4096 cube draws, marks the sim CEO_SAMPLER_FAIL.
+size_t ceo_device_sample_q(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
+                           double radius, double min_sep, uint32_t max_attempts,
+                           int32_t charge_levels, int32_t max_total, double* pos_out,
+                           double* q_out, uint8_t* status_out);
+
 size_t ceo_device_sample(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
                          double radius, double min_sep, uint32_t max_attempts, double* pos_out,
                          uint8_t* status_out) {
+  return ceo_device_sample_q(seed, first_index, n_sims, n_atoms, radius, min_sep, max_attempts,
+                             0, 0, pos_out, nullptr, status_out);
+}
+
+// As ceo_device_sample, plus per-sim charges when q_out is given: try t draws
+// Philox4x32-10 on counters (4t + i/4, lo32(g), hi32(g), 1), word i % 4, and
+// q_i = 1 + ((u24 * levels) >> 24); tries continue while the sum exceeds
+// max_total (> 0), at most 4096.
+size_t ceo_device_sample_q(uint64_t seed, uint64_t first_index, size_t n_sims, int32_t n_atoms,
+                           double radius, double min_sep, uint32_t max_attempts,
+                           int32_t charge_levels, int32_t max_total, double* pos_out,
+                           double* q_out, uint8_t* status_out) {
   const uint32_t key[2] = {static_cast<uint32_t>(seed), static_cast<uint32_t>(seed >> 32)};
   const float R = static_cast<float>(radius);
   const float twoR = static_cast<float>(2.0 * radius);
@@ -497,6 +561,30 @@ size_t ceo_device_sample(uint64_t seed, uint64_t first_index, size_t n_sims, int
           break;
         }
       }
+    }
+    if (q_out) {
+      double Q[kMaxN];
+      bool placed = false;
+      for (uint32_t t = 0; t < 4096 && !placed; ++t) {
+        int total = 0;
+        for (int b = 0; b < (n_atoms + 3) / 4; ++b) {
+          const uint32_t ctr[4] = {4u * t + static_cast<uint32_t>(b), static_cast<uint32_t>(g),
+                                   static_cast<uint32_t>(g >> 32), 1u};
+          uint32_t r[4];
+          ceo_philox4x32_10(ctr, key, r);
+          for (int w = 0; w < 4; ++w) {
+            const int i = 4 * b + w;
+            if (i >= n_atoms) break;
+            const int qi = 1 + static_cast<int>((static_cast<uint64_t>(r[w] >> 8) *
+                                                 static_cast<uint32_t>(charge_levels)) >> 24);
+            Q[i] = qi;
+            total += qi;
+          }
+        }
+        placed = max_total <= 0 || total <= max_total;
+      }
+      ok_sim = ok_sim && placed;
+      for (int i = 0; i < n_atoms; ++i) q_out[s * n_atoms + i] = ok_sim ? Q[i] : 0.0;
     }
     double* out = pos_out + s * n_atoms * 3;
     for (int q = 0; q < 3 * n_atoms; ++q) out[q] = ok_sim ? static_cast<double>(P[q]) : 0.0;

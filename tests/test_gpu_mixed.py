@@ -1,0 +1,101 @@
+"""Mixed-size launches with per-sim charges (the N2 / CO2 / CH4 / C2H6 workload),
+through the C-ABI, against the oracle's per-lane statement with the same
+geometries and charges."""
+import numpy as np
+import pytest
+
+from conftest import per_atom_rel
+
+pytestmark = pytest.mark.gpu
+
+C, O, H, N14 = 21874.66, 29156.95, 1837.15, 25526.04
+MIXED = [  # (masses, radius = n^(1/3) + 1.5)
+    ([N14, N14], 2 ** (1 / 3) + 1.5),
+    ([C, O, O], 3 ** (1 / 3) + 1.5),
+    ([C, H, H, H, H], 5 ** (1 / 3) + 1.5),
+    ([C, C] + [H] * 6, 8 ** (1 / 3) + 1.5),
+]
+RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+
+
+def species(ce, count, masses_radii=MIXED, levels=2):
+    out = []
+    for masses, R in masses_radii:
+        mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in masses])
+        out.append(ce.Species(mol, count, R, 0.9, charge_levels=levels,
+                              max_total_charge=2 * len(masses)))
+    return out
+
+
+def test_species_sampler_bit_exact(gpu, oracle):
+    ce = gpu
+    sp = species(ce, 700)
+    mb = ce.MixedBatch(sp, 1.0, 1e-6, 1e-9)
+    mb.sample(seed=77, first_index=1000)
+    geoms, charges = mb.geometries(), mb.charges()
+    start = 1000
+    for s, g, q in zip(sp, geoms, charges):
+        n = s.molecule.size()
+        ref_g, ref_q, st, fails = oracle.device_sample_q(77, start, s.count, n, s.radius, 0.9,
+                                                         2, 2 * n)
+        assert fails == 0
+        np.testing.assert_array_equal(g, ref_g)
+        np.testing.assert_array_equal(q, ref_q)
+        assert set(np.unique(q)) <= {1.0, 2.0}
+        start += s.count
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-12)])
+def test_mixed_launch_vs_oracle(gpu, oracle, precision, tol):
+    ce = gpu
+    from oracle import ffi
+    sp = species(ce, 600)
+    rtol, atol = (1e-6, 1e-9) if precision == 0 else (1e-10, 1e-14)
+    mb = ce.MixedBatch(sp, 1.0, rtol, atol)
+    mb.sample(seed=5)
+    res = mb.run(ce.RunConfig(precision=precision, **RULE))
+    assert mb.stats()["launches"] == 1  # sizes (2, 3, 5, 8): one persistent launch
+    for s, g, q, r in zip(sp, mb.geometries(), mb.charges(), res):
+        ref = oracle.lane_run_q(s.molecule.masses, q, g,
+                                ffi.make_config(rtol=rtol, atol=atol, **RULE),
+                                precision="f32" if precision == 0 else "f64")
+        assert per_atom_rel(r["momenta"], ref.momenta).max() < tol
+        np.testing.assert_array_equal(r["status"], ref.status)
+        if precision == 1:
+            np.testing.assert_array_equal(r["steps"], ref.steps)
+
+
+def test_mixed_equals_single_size_launches(gpu):
+    """The one-launch mixed kernel gives the same bits as one engine per size."""
+    ce = gpu
+    sp = species(ce, 500)
+    mb = ce.MixedBatch(sp, 1.0, 1e-6, 1e-9)
+    mb.sample(seed=9)
+    cfg = ce.RunConfig(**RULE)
+    res = mb.run(cfg)
+    for s, g, q, r in zip(sp, mb.geometries(), mb.charges(), res):
+        eng = ce.BatchedRK45(s.molecule, 1.0, s.count, 1e-6, 1e-9)
+        eng.set_geometries(g)
+        eng.set_charges(q)
+        one = eng.run(cfg)
+        np.testing.assert_array_equal(one.momenta.reshape(r["momenta"].shape), r["momenta"])
+        np.testing.assert_array_equal(one.steps, r["steps"])
+
+
+def test_other_size_sets_run_per_size(gpu, oracle):
+    """Sizes outside the compiled mixed set run one launch per species."""
+    ce = gpu
+    from oracle import ffi
+    n10 = [C, C, O, O] + [H] * 6
+    sp = [ce.Species(ce.Molecule([ce.Atom("X", m, 1.0) for m in n10]), 300, 3.5, 0.9, 3),
+          ce.Species(ce.Molecule([ce.Atom("X", m, 1.0) for m in [C] + [H] * 4]), 200, 2.5, 1.0)]
+    mb = ce.MixedBatch(sp, 1.0, 1e-6, 1e-9)
+    mb.sample(seed=21)
+    res = mb.run(ce.RunConfig(**RULE))
+    assert mb.stats()["launches"] == 2
+    q10 = mb.charges()[0]
+    assert set(np.unique(q10)) <= {1.0, 2.0, 3.0} and len(np.unique(q10)) == 3
+    for s, g, q, r in zip(sp, mb.geometries(), mb.charges(), res):
+        ref = oracle.lane_run_q(s.molecule.masses, q, g, ffi.make_config(rtol=1e-6, atol=1e-9,
+                                                                         **RULE))
+        assert per_atom_rel(r["momenta"], ref.momenta).max() < 1e-5
