@@ -124,6 +124,16 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_host_sampler_destroy": ([P], C.c_int),
         "ce_batched_accelerations_f32": ([S, S, P, P, P, P, P, P, P, P], C.c_int),
         "ce_batched_potential_energy_f32": ([S, S, P, P, P, P, P], C.c_int),
+        "ce_mlp_create": ([P, C.c_int32, C.c_uint64, P], C.c_int),
+        "ce_mlp_destroy": ([P], C.c_int),
+        "ce_mlp_param_count": ([P], C.c_int64),
+        "ce_mlp_get_params": ([P, P], C.c_int),
+        "ce_mlp_set_params": ([P, P], C.c_int),
+        "ce_mlp_forward": ([P, P, S, P], C.c_int),
+        "ce_mlp_gradients": ([P, P, P, S, P, P], C.c_int),
+        "ce_mlp_adam_step": ([P, C.c_float], C.c_int),
+        "ce_mlp_set_dataset": ([P, P, P, S], C.c_int),
+        "ce_mlp_train_epoch": ([P, S, C.c_float, C.c_uint64, P], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -552,3 +562,73 @@ def batched_potential_energy(n: int, k: int, px, py, pz, pair_const):
     _check(_lib.ce_batched_potential_energy_f32(n, k, *[a.ctypes.data for a in arrs],
                                                 pe.ctypes.data))
     return pe
+
+
+class MLP:
+    """The closure network on the GPU: SiLU MLP with BF16 MFMA GEMMs (FP32
+    accumulation), FP32 master parameters, MSE loss and Adam."""
+
+    def __init__(self, widths, seed: int = 0):
+        self.widths = [int(w) for w in widths]
+        w = np.asarray(self.widths, dtype=np.int32)
+        h = C.c_void_p()
+        _check(_lib.ce_mlp_create(w.ctypes.data, len(w), seed, C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.ce_mlp_destroy(self._h)
+            self._h = None
+
+    @property
+    def n_params(self) -> int:
+        return int(_lib.ce_mlp_param_count(self._h))
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        _check(_lib.ce_mlp_get_params(self._h, out.ctypes.data))
+        return out
+
+    def set_params(self, p) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {p.size}")
+        _check(_lib.ce_mlp_set_params(self._h, p.ctypes.data))
+
+    def _rows(self, x, width):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        if x.ndim != 2 or x.shape[1] != width:
+            raise ValueError(f"expected [batch][{width}] array, got {x.shape}")
+        return x
+
+    def forward(self, x) -> np.ndarray:
+        x = self._rows(x, self.widths[0])
+        y = np.empty((x.shape[0], self.widths[-1]), dtype=np.float32)
+        _check(_lib.ce_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
+    def gradients(self, x, t):
+        x = self._rows(x, self.widths[0])
+        t = self._rows(t, self.widths[-1])
+        if t.shape[0] != x.shape[0]:
+            raise ValueError("x and t differ in batch size")
+        g = np.empty(self.n_params, dtype=np.float32)
+        loss = C.c_float(0.0)
+        _check(_lib.ce_mlp_gradients(self._h, x.ctypes.data, t.ctypes.data, x.shape[0],
+                                     g.ctypes.data, C.byref(loss)))
+        return float(loss.value), g
+
+    def adam_step(self, lr: float) -> None:
+        _check(_lib.ce_mlp_adam_step(self._h, lr))
+
+    def set_dataset(self, x, t) -> None:
+        x = self._rows(x, self.widths[0])
+        t = self._rows(t, self.widths[-1])
+        if t.shape[0] != x.shape[0]:
+            raise ValueError("x and t differ in length")
+        _check(_lib.ce_mlp_set_dataset(self._h, x.ctypes.data, t.ctypes.data, x.shape[0]))
+
+    def train_epoch(self, batch: int, lr: float, shuffle_seed: int = 0) -> float:
+        loss = C.c_float(0.0)
+        _check(_lib.ce_mlp_train_epoch(self._h, batch, lr, shuffle_seed, C.byref(loss)))
+        return float(loss.value)

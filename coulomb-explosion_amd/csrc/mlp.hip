@@ -1,0 +1,646 @@
+// Closure network: a small MLP (SiLU hidden layers) trained and evaluated on the
+// GPU with hand-written BF16 MFMA GEMMs accumulating in FP32 (gfx950
+// v_mfma_f32_32x32x16_bf16), FP32 master weights and Adam.
+//
+// Layouts.  Activations are row-major [batch][features] in bf16.  Layer l has
+// W_l [out][in] (bf16 copy of the FP32 master, refreshed after every update) and
+// bias b_l [out] (FP32).  Parameters are stored W_0, b_0, W_1, b_1, ... in one
+// FP32 vector.
+//   forward   Z = X W^T + b,  Y = silu(Z)     (A = X, B(k=i, n=o) = W[o][i])
+//   backward  dZ = dY * silu'(Z)
+//             dX = dZ W                        (A = dZ, B(k=o, n=i) = W[o][i])
+//             dW = dZ^T X,  db = sum_b dZ      (A(m=o, k=b) = dZ[b][o], B(k=b, n=i) = X[b][i])
+// The GEMM takes general element strides for A and B and stages 64 x 32 / 32 x 64
+// tiles in LDS with k contiguous, so every product above uses the same kernel.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "coulomb_explosion.h"
+
+namespace ce {
+int set_error(int code, const std::string& msg);
+}
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
+  // round to nearest even (inputs are finite in this network)
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return static_cast<uint16_t>(u >> 16);
+}
+__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+constexpr int kTM = 64, kTN = 64, kTK = 32;
+enum Epilogue : int { kEpiStoreF32 = 0, kEpiBiasSiluBf16 = 1, kEpiBiasF32 = 2, kEpiBf16 = 3 };
+
+struct GemmArgs {
+  const uint16_t* A;  // bf16 bits
+  const uint16_t* B;
+  int64_t sAm, sAk, sBk, sBn;  // element strides
+  int M, N, K;                 // K may exceed the stored extent only with zero padding
+  const float* bias;           // [N] or nullptr
+  float* C32;                  // fp32 out [M][ldc] (pre-activation / plain)
+  uint16_t* C16;               // bf16 out [M][ldc]
+  int64_t ldc;
+  int epilogue;
+  int accumulate;  // C32 += result
+};
+
+// One workgroup = 4 waves = a 64 x 64 output tile; wave w owns the 32 x 32 block
+// (w / 2, w % 2).  Per 32-deep K step: stage A (64 x 32) as As[m][k] and B
+// (32 x 64) as Bs[n][k] in LDS, then two v_mfma_f32_32x32x16_bf16 per wave.
+// MFMA operand layout (32x32x16): lane l holds A[l % 32][8 (l / 32) + 0..7] and
+// B[8 (l / 32) + 0..7][l % 32]; the result register r of lane l is
+// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+__global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
+  __shared__ __align__(16) uint16_t As[kTM][kTK + 8];
+  __shared__ __align__(16) uint16_t Bs[kTN][kTK + 8];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+  for (int k0 = 0; k0 < g.K; k0 += kTK) {
+    // 64 x 32 elements of A and 32 x 64 of B, 8 per thread each.
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int idx = tid * 8 + e;  // 0..2047
+      const int r = idx / kTK, c = idx % kTK;
+      const int m = m0 + r, k = k0 + c;
+      As[r][c] = (m < g.M && k < g.K) ? g.A[m * g.sAm + k * g.sAk] : uint16_t(0);
+      const int n = n0 + r;
+      Bs[r][c] = (n < g.N && k < g.K) ? g.B[k * g.sBk + n * g.sBn] : uint16_t(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < kTK; kk += 16) {
+      const int row = lane & 31, kq = kk + 8 * (lane >> 5);
+      bf16x8 a, b;
+      const uint16_t* pa = &As[wm + row][kq];
+      const uint16_t* pb = &Bs[wn + row][kq];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        a[j] = __builtin_bit_cast(__bf16, pa[j]);
+        b[j] = __builtin_bit_cast(__bf16, pb[j]);
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int col = n0 + wn + (lane & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (row >= g.M || col >= g.N) continue;
+    float v = acc[r];
+    if (g.bias) v += g.bias[col];
+    const int64_t o = row * g.ldc + col;
+    switch (g.epilogue) {
+      case kEpiStoreF32:
+      case kEpiBiasF32:
+        g.C32[o] = g.accumulate ? g.C32[o] + v : v;
+        break;
+      case kEpiBiasSiluBf16:
+        if (g.C32) g.C32[o] = v;
+        g.C16[o] = f32_to_bf16_bits(silu(v));
+        break;
+      case kEpiBf16:
+        g.C16[o] = f32_to_bf16_bits(v);
+        break;
+    }
+  }
+}
+
+// elementwise helpers -------------------------------------------------------
+__global__ void to_bf16(const float* x, uint16_t* y, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < n) y[i] = f32_to_bf16_bits(x[i]);
+}
+
+// dZ = dY * silu'(Z) (bf16 out), dY bf16/fp32 in
+__global__ void silu_backward(const float* dY, const float* Z, int64_t n, uint16_t* dZ16,
+                              float* dZ32) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const float d = dY[i] * silu_grad(Z[i]);
+  dZ16[i] = f32_to_bf16_bits(d);
+  if (dZ32) dZ32[i] = d;
+}
+
+// column sums of a [rows][cols] fp32 matrix (bias gradients): one block per column.
+__global__ void column_sums(const float* x, int64_t rows, int cols, int64_t ld, float* out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.0f;
+  for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) s += x[r * ld + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// MSE loss on the output layer: dY = 2 (Y - T) / (rows * cols), loss partials.
+__global__ void mse_grad(const float* Y, int64_t ldy, const float* T, int64_t rows, int cols,
+                         float* dY, float* loss_partial) {
+  __shared__ float red[256];
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  float l = 0.0f;
+  if (i < rows * cols) {
+    const int64_t r = i / cols;
+    const int c = static_cast<int>(i % cols);
+    const float d = Y[r * ldy + c] - T[i];
+    dY[r * ldy + c] = 2.0f * d / static_cast<float>(rows * cols);
+    l = d * d;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(loss_partial, red[0] / static_cast<float>(rows * cols));
+}
+
+// Adam (Kingma & Ba) on fp32 master parameters.
+__global__ void adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                     float b2, float eps, float c1, float c2) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host ----
+
+struct ce_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<int> widths;  // w[0] inputs ... w[L] outputs
+  int L = 0;
+  std::vector<int64_t> w_off, b_off;  // offsets into the parameter vector
+  int64_t n_params = 0;
+  float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;  // master, grad, Adam moments
+  std::vector<uint16_t*> Wb;  // bf16 copies [out][in]
+  int64_t adam_t = 0;
+  // batch workspace
+  int64_t cap = 0;
+  std::vector<uint16_t*> act;  // act[l]: bf16 input of layer l [cap][w[l]]
+  std::vector<float*> Z;       // Z[l]: pre-activation of hidden layer l [cap][w[l+1]]
+  float* out = nullptr;        // [cap][w[L]]
+  float *dY = nullptr, *dZ32 = nullptr;
+  uint16_t* dZ16 = nullptr;
+  float* dX = nullptr;
+  float* loss = nullptr;
+  float* xin = nullptr;  // fp32 staging of inputs [cap][w[0]]
+  float* tgt = nullptr;  // fp32 targets [cap][w[L]]
+  // device-resident training set
+  float *ds_x = nullptr, *ds_t = nullptr;
+  int64_t ds_n = 0;
+  int64_t* perm = nullptr;
+};
+
+namespace {
+
+int fail(int code, const std::string& m) { return ce::set_error(code, m); }
+
+#define MLP_HIP(call)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, int64_t n) {
+  *p = nullptr;
+  MLP_HIP(hipMalloc(reinterpret_cast<void**>(p), (n > 0 ? n : 1) * sizeof(T)));
+  return CE_OK;
+}
+
+unsigned blocks_for(int64_t n, int t = 256) { return static_cast<unsigned>((n + t - 1) / t); }
+
+void free_mlp(ce_mlp* m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16), f(m->dX);
+  f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
+  for (auto* p : m->Wb) f(p);
+  for (auto* p : m->act) f(p);
+  for (auto* p : m->Z) f(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+int ensure_batch(ce_mlp* m, int64_t B) {
+  if (B <= m->cap) return CE_OK;
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  for (auto* p : m->act) f(p);
+  for (auto* p : m->Z) f(p);
+  f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16), f(m->dX), f(m->xin), f(m->tgt);
+  m->act.assign(m->L, nullptr);
+  m->Z.assign(m->L, nullptr);
+  int maxw = 0;
+  for (int w : m->widths) maxw = w > maxw ? w : maxw;
+  int rc = CE_OK;
+  for (int l = 0; l < m->L && rc == CE_OK; ++l) {
+    rc = dalloc(&m->act[l], B * m->widths[l]);
+    if (rc == CE_OK && l + 1 < m->L) rc = dalloc(&m->Z[l], B * m->widths[l + 1]);
+  }
+  if (rc == CE_OK) rc = dalloc(&m->out, B * m->widths[m->L]);
+  if (rc == CE_OK) rc = dalloc(&m->dY, B * maxw);
+  if (rc == CE_OK) rc = dalloc(&m->dZ32, B * maxw);
+  if (rc == CE_OK) rc = dalloc(&m->dZ16, B * maxw);
+  if (rc == CE_OK) rc = dalloc(&m->dX, B * maxw);
+  if (rc == CE_OK) rc = dalloc(&m->xin, B * m->widths[0]);
+  if (rc == CE_OK) rc = dalloc(&m->tgt, B * m->widths[m->L]);
+  if (rc == CE_OK) m->cap = B;
+  return rc;
+}
+
+void gemm(hipStream_t s, const GemmArgs& g) {
+  const dim3 grid(blocks_for(g.N, kTN), blocks_for(g.M, kTM));
+  hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, s, g);
+}
+
+int export_all(ce_mlp* m) {
+  for (int l = 0; l < m->L; ++l) {
+    const int in = m->widths[l], out = m->widths[l + 1];
+    const int64_t n = static_cast<int64_t>(in) * out;
+    hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, m->stream, m->P + m->w_off[l],
+                       m->Wb[l], n);
+  }
+  MLP_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+// Forward pass of the batch staged in m->xin (fp32 [B][w0]); the output lands in
+// m->out (fp32 [B][wL]).  Keeps Z and the bf16 activations for backward.
+int forward(ce_mlp* m, int64_t B, bool keep) {
+  const int w0 = m->widths[0];
+  hipLaunchKernelGGL(to_bf16, dim3(blocks_for(B * w0)), dim3(256), 0, m->stream, m->xin,
+                     m->act[0], B * w0);
+  for (int l = 0; l < m->L; ++l) {
+    const int in = m->widths[l], out = m->widths[l + 1];
+    GemmArgs g{};
+    g.A = m->act[l];
+    g.sAm = in;
+    g.sAk = 1;
+    g.B = m->Wb[l];
+    g.sBk = 1;
+    g.sBn = in;
+    g.M = static_cast<int>(B);
+    g.N = out;
+    g.K = in;
+    g.bias = m->P + m->b_off[l];
+    g.ldc = out;
+    if (l + 1 < m->L) {
+      g.epilogue = kEpiBiasSiluBf16;
+      g.C16 = m->act[l + 1];
+      g.C32 = keep ? m->Z[l] : nullptr;
+    } else {
+      g.epilogue = kEpiBiasF32;
+      g.C32 = m->out;
+    }
+    gemm(m->stream, g);
+  }
+  MLP_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+// Backward of the last forward (keep = true), from dL/dout in m->dY [B][wL];
+// writes parameter gradients into m->G.
+int backward(ce_mlp* m, int64_t B) {
+  float* dY = m->dY;
+  for (int l = m->L - 1; l >= 0; --l) {
+    const int in = m->widths[l], out = m->widths[l + 1];
+    const int64_t n = B * out;
+    if (l + 1 < m->L) {
+      hipLaunchKernelGGL(silu_backward, dim3(blocks_for(n)), dim3(256), 0, m->stream, dY, m->Z[l],
+                         n, m->dZ16, m->dZ32);
+    } else {
+      hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, m->stream, dY, m->dZ16, n);
+      MLP_HIP(hipMemcpyAsync(m->dZ32, dY, n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+    }
+    hipLaunchKernelGGL(column_sums, dim3(out), dim3(256), 0, m->stream, m->dZ32, B, out,
+                       static_cast<int64_t>(out), m->G + m->b_off[l]);
+    GemmArgs gw{};  // dW[o][i] = sum_b dZ[b][o] X[b][i]
+    gw.A = m->dZ16;
+    gw.sAm = 1;
+    gw.sAk = out;
+    gw.B = m->act[l];
+    gw.sBk = in;
+    gw.sBn = 1;
+    gw.M = out;
+    gw.N = in;
+    gw.K = static_cast<int>(B);
+    gw.C32 = m->G + m->w_off[l];
+    gw.ldc = in;
+    gw.epilogue = kEpiStoreF32;
+    gemm(m->stream, gw);
+    if (l > 0) {  // dX[b][i] = sum_o dZ[b][o] W[o][i]
+      GemmArgs gx{};
+      gx.A = m->dZ16;
+      gx.sAm = out;
+      gx.sAk = 1;
+      gx.B = m->Wb[l];
+      gx.sBk = in;
+      gx.sBn = 1;
+      gx.M = static_cast<int>(B);
+      gx.N = in;
+      gx.K = out;
+      gx.C32 = m->dX;
+      gx.ldc = in;
+      gx.epilogue = kEpiStoreF32;
+      gemm(m->stream, gx);
+      MLP_HIP(hipMemcpyAsync(m->dY, m->dX, B * in * sizeof(float), hipMemcpyDeviceToDevice,
+                             m->stream));
+      dY = m->dY;
+    }
+  }
+  MLP_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+// deterministic init on the host: U(-sqrt(6/(in+out)), +) (Glorot), zero biases
+void init_params(const ce_mlp* m, uint64_t seed, std::vector<float>& p) {
+  p.assign(m->n_params, 0.0f);
+  uint64_t s = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  auto next = [&s]() {  // splitmix64
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  for (int l = 0; l < m->L; ++l) {
+    const int in = m->widths[l], out = m->widths[l + 1];
+    const float lim = std::sqrt(6.0f / static_cast<float>(in + out));
+    for (int64_t i = 0; i < static_cast<int64_t>(in) * out; ++i) {
+      const double u = (next() >> 11) * (1.0 / 9007199254740992.0);
+      p[m->w_off[l] + i] = static_cast<float>((2.0 * u - 1.0) * lim);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp** out) {
+  if (out == nullptr || widths == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (n_widths < 2 || n_widths > 16) return fail(CE_ERR_INVALID_ARGUMENT, "2..16 widths");
+  for (int i = 0; i < n_widths; ++i)
+    if (widths[i] < 1 || widths[i] > 4096) return fail(CE_ERR_INVALID_ARGUMENT, "width out of range");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CE_ERR_NO_DEVICE, "no HIP device");
+  ce_mlp* m = new ce_mlp();
+  m->widths.assign(widths, widths + n_widths);
+  m->L = n_widths - 1;
+  for (int l = 0; l < m->L; ++l) {
+    m->w_off.push_back(m->n_params);
+    m->n_params += static_cast<int64_t>(widths[l]) * widths[l + 1];
+    m->b_off.push_back(m->n_params);
+    m->n_params += widths[l + 1];
+  }
+  int rc = CE_OK;
+  if (hipGetDevice(&m->device) != hipSuccess || hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(CE_ERR_HIP, "stream");
+  if (rc == CE_OK) rc = dalloc(&m->P, m->n_params);
+  if (rc == CE_OK) rc = dalloc(&m->G, m->n_params);
+  if (rc == CE_OK) rc = dalloc(&m->M1, m->n_params);
+  if (rc == CE_OK) rc = dalloc(&m->M2, m->n_params);
+  if (rc == CE_OK) rc = dalloc(&m->loss, 1);
+  m->Wb.assign(m->L, nullptr);
+  for (int l = 0; l < m->L && rc == CE_OK; ++l)
+    rc = dalloc(&m->Wb[l], static_cast<int64_t>(widths[l]) * widths[l + 1]);
+  if (rc == CE_OK) {
+    std::vector<float> p;
+    init_params(m, seed, p);
+    if (hipMemcpyAsync(m->P, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice,
+                       m->stream) != hipSuccess ||
+        hipMemsetAsync(m->M1, 0, m->n_params * sizeof(float), m->stream) != hipSuccess ||
+        hipMemsetAsync(m->M2, 0, m->n_params * sizeof(float), m->stream) != hipSuccess ||
+        hipStreamSynchronize(m->stream) != hipSuccess)
+      rc = fail(CE_ERR_HIP, "parameter upload");
+  }
+  if (rc == CE_OK) rc = export_all(m);
+  if (rc == CE_OK && hipStreamSynchronize(m->stream) != hipSuccess) rc = fail(CE_ERR_HIP, "sync");
+  if (rc != CE_OK) {
+    free_mlp(m);
+    return rc;
+  }
+  *out = m;
+  return CE_OK;
+}
+
+int ce_mlp_destroy(ce_mlp* m) {
+  free_mlp(m);
+  return CE_OK;
+}
+
+int64_t ce_mlp_param_count(const ce_mlp* m) { return m ? m->n_params : 0; }
+
+int ce_mlp_get_params(ce_mlp* m, float* host) {
+  if (!m || !host) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipMemcpyAsync(host, m->P, m->n_params * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return CE_OK;
+}
+
+int ce_mlp_set_params(ce_mlp* m, const float* host) {
+  if (!m || !host) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipMemcpyAsync(m->P, host, m->n_params * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  m->adam_t = 0;
+  MLP_HIP(hipMemsetAsync(m->M1, 0, m->n_params * sizeof(float), m->stream));
+  MLP_HIP(hipMemsetAsync(m->M2, 0, m->n_params * sizeof(float), m->stream));
+  const int rc = export_all(m);
+  if (rc != CE_OK) return rc;
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return CE_OK;
+}
+
+int ce_mlp_forward(ce_mlp* m, const float* x_host, size_t batch, float* y_host) {
+  if (!m || (!x_host && batch) || (!y_host && batch)) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  const int64_t chunk = 65536;
+  for (size_t b0 = 0; b0 < batch; b0 += chunk) {
+    const int64_t B = static_cast<int64_t>(std::min<size_t>(chunk, batch - b0));
+    int rc = ensure_batch(m, B);
+    if (rc != CE_OK) return rc;
+    MLP_HIP(hipMemcpyAsync(m->xin, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
+                           hipMemcpyHostToDevice, m->stream));
+    rc = forward(m, B, false);
+    if (rc != CE_OK) return rc;
+    MLP_HIP(hipMemcpyAsync(y_host + b0 * m->widths[m->L], m->out,
+                           B * m->widths[m->L] * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  }
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return CE_OK;
+}
+
+// Loss (mean squared error over the batch and outputs) and its gradient with
+// respect to every parameter, for the batch given on the host.
+int ce_mlp_gradients(ce_mlp* m, const float* x_host, const float* t_host, size_t batch,
+                     float* grad_host, float* loss_host) {
+  if (!m || !x_host || !t_host || !grad_host || batch == 0)
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument or empty batch");
+  MLP_HIP(hipSetDevice(m->device));
+  const int64_t B = static_cast<int64_t>(batch);
+  int rc = ensure_batch(m, B);
+  if (rc != CE_OK) return rc;
+  const int wo = m->widths[m->L];
+  MLP_HIP(hipMemcpyAsync(m->xin, x_host, B * m->widths[0] * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(hipMemcpyAsync(m->tgt, t_host, B * wo * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  rc = forward(m, B, true);
+  if (rc != CE_OK) return rc;
+  MLP_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
+  hipLaunchKernelGGL(mse_grad, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->out,
+                     static_cast<int64_t>(wo), m->tgt, B, wo, m->dY, m->loss);
+  rc = backward(m, B);
+  if (rc != CE_OK) return rc;
+  MLP_HIP(hipMemcpyAsync(grad_host, m->G, m->n_params * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  if (loss_host) MLP_HIP(hipMemcpyAsync(loss_host, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return CE_OK;
+}
+
+int ce_mlp_set_dataset(ce_mlp* m, const float* x_host, const float* t_host, size_t n) {
+  if (!m || (!x_host && n) || (!t_host && n)) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  auto f = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  f(m->ds_x), f(m->ds_t), f(m->perm);
+  m->ds_x = m->ds_t = nullptr;
+  m->perm = nullptr;
+  m->ds_n = 0;
+  int rc = dalloc(&m->ds_x, static_cast<int64_t>(n) * m->widths[0]);
+  if (rc == CE_OK) rc = dalloc(&m->ds_t, static_cast<int64_t>(n) * m->widths[m->L]);
+  if (rc == CE_OK) rc = dalloc(&m->perm, static_cast<int64_t>(n));
+  if (rc != CE_OK) return rc;
+  MLP_HIP(hipMemcpyAsync(m->ds_x, x_host, n * m->widths[0] * sizeof(float), hipMemcpyHostToDevice,
+                         m->stream));
+  MLP_HIP(hipMemcpyAsync(m->ds_t, t_host, n * m->widths[m->L] * sizeof(float),
+                         hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  m->ds_n = static_cast<int64_t>(n);
+  return CE_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void gather_rows(const float* src, const int64_t* idx, int64_t rows, int cols, float* dst) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= rows * cols) return;
+  const int64_t r = i / cols;
+  const int c = static_cast<int>(i % cols);
+  dst[i] = src[idx[r] * cols + c];
+}
+
+int adam_step(ce_mlp* m, float lr) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  m->adam_t += 1;
+  const float c1 = 1.0f - std::pow(b1, static_cast<float>(m->adam_t));
+  const float c2 = 1.0f - std::pow(b2, static_cast<float>(m->adam_t));
+  hipLaunchKernelGGL(adam, dim3(blocks_for(m->n_params)), dim3(256), 0, m->stream, m->P, m->G,
+                     m->M1, m->M2, m->n_params, lr, b1, b2, eps, c1, c2);
+  MLP_HIP(hipGetLastError());
+  return export_all(m);
+}
+
+}  // namespace
+
+extern "C" {
+
+// One pass over the device-resident training set in a shuffled order (the
+// permutation comes from the host, seeded), Adam after every batch.
+int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed, float* mean_loss) {
+  if (!m || batch == 0) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument or zero batch");
+  if (m->ds_n == 0) return fail(CE_ERR_RUNTIME, "no training set: call ce_mlp_set_dataset");
+  MLP_HIP(hipSetDevice(m->device));
+  const int64_t n = m->ds_n, B = static_cast<int64_t>(batch);
+  int rc = ensure_batch(m, B);
+  if (rc != CE_OK) return rc;
+  std::vector<int64_t> perm(n);
+  for (int64_t i = 0; i < n; ++i) perm[i] = i;
+  uint64_t s = shuffle_seed * 0x9E3779B97F4A7C15ull + 1;
+  for (int64_t i = n - 1; i > 0; --i) {  // Fisher-Yates with splitmix64
+    s += 0x9E3779B97F4A7C15ull;
+    uint64_t z = s;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const int64_t j = static_cast<int64_t>(z % static_cast<uint64_t>(i + 1));
+    std::swap(perm[i], perm[j]);
+  }
+  MLP_HIP(hipMemcpyAsync(m->perm, perm.data(), n * sizeof(int64_t), hipMemcpyHostToDevice,
+                         m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));  // perm is a local host buffer
+  const int wi = m->widths[0], wo = m->widths[m->L];
+  MLP_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
+  int64_t n_batches = 0;
+  for (int64_t b0 = 0; b0 + B <= n; b0 += B, ++n_batches) {  // whole batches only
+    hipLaunchKernelGGL(gather_rows, dim3(blocks_for(B * wi)), dim3(256), 0, m->stream, m->ds_x,
+                       m->perm + b0, B, wi, m->xin);
+    hipLaunchKernelGGL(gather_rows, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->ds_t,
+                       m->perm + b0, B, wo, m->tgt);
+    rc = forward(m, B, true);
+    if (rc != CE_OK) return rc;
+    hipLaunchKernelGGL(mse_grad, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->out,
+                       static_cast<int64_t>(wo), m->tgt, B, wo, m->dY, m->loss);
+    rc = backward(m, B);
+    if (rc == CE_OK) rc = adam_step(m, lr);
+    if (rc != CE_OK) return rc;
+  }
+  float total = 0.0f;
+  MLP_HIP(hipMemcpyAsync(&total, m->loss, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  if (mean_loss) *mean_loss = n_batches ? total / static_cast<float>(n_batches) : 0.0f;
+  return CE_OK;
+}
+
+// One Adam step on the gradients left by ce_mlp_gradients (tests).
+int ce_mlp_adam_step(ce_mlp* m, float lr) {
+  if (!m) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  const int rc = adam_step(m, lr);
+  if (rc != CE_OK) return rc;
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return CE_OK;
+}
+
+}  // extern "C"

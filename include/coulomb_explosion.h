@@ -224,6 +224,30 @@ size_t ce_record_bytes(int32_t n_atoms);
  * and copies them to host_out (n_sims * ce_record_bytes(n) bytes). */
 int ce_engine_pack_records(ce_engine* eng, uint64_t first, size_t n_sims, uint8_t* host_out);
 
+/* ---- closure network (no reference counterpart) --------------------------
+ * A multilayer perceptron: widths[0] inputs, SiLU hidden layers, widths[n-1]
+ * linear outputs; parameters W_0 [w1][w0], b_0 [w1], W_1, b_1, ... in one FP32
+ * vector (Glorot-uniform weights from `seed`, zero biases).  Forward and backward
+ * run as BF16 MFMA GEMMs with FP32 accumulation; Adam (beta 0.9 / 0.999,
+ * eps 1e-8) updates FP32 master parameters.  Loss: mean squared error. */
+typedef struct ce_mlp ce_mlp;
+int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp** out);
+int ce_mlp_destroy(ce_mlp* m);
+int64_t ce_mlp_param_count(const ce_mlp* m);
+int ce_mlp_get_params(ce_mlp* m, float* params);
+int ce_mlp_set_params(ce_mlp* m, const float* params);  /* also resets Adam */
+/* y [batch][w_last] = net(x [batch][w0]); host buffers. */
+int ce_mlp_forward(ce_mlp* m, const float* x, size_t batch, float* y);
+/* Loss and d loss / d params for one batch (host buffers). */
+int ce_mlp_gradients(ce_mlp* m, const float* x, const float* t, size_t batch, float* grad,
+                     float* loss);
+/* One Adam step with the gradients of the last ce_mlp_gradients call. */
+int ce_mlp_adam_step(ce_mlp* m, float lr);
+/* Device-resident training set and one shuffled epoch of Adam steps over it. */
+int ce_mlp_set_dataset(ce_mlp* m, const float* x, const float* t, size_t n);
+int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed,
+                       float* mean_loss);
+
 /* Host sampler with the reference's exact stream: UniformSphereSampler(Options,
  * seed) (sampler.hpp:46-72, sampler.cpp:7-57) -- std::mt19937_64 feeding
  * std::uniform_real_distribution<double>(-R, R), cube rejection, sequential
