@@ -17,6 +17,12 @@ Other workloads (--workload):
          {1, 2}; R = N^(1/3) + 1.5, min sep 0.9; same tolerances and stop rule.
   n10    10M sims of a 10-atom molecule (2 C, 2 O, 6 H), charges uniform in
          {1, 2, 3}, R 3.5, min sep 0.9 (the plain method on that distribution).
+  closure  the learned closure on CH4: an MLP ensemble (40 -> 256 x 3 -> 15,
+         SiLU, BF16 MFMA / FP32 accumulation, Adam lr 1e-3, batch 8192, 20
+         epochs) trained on 2M engine trajectories (untimed), then 8M fresh sims
+         per step: plain DP5(4) to t_cut = 200, prior + ensemble residual where
+         the members agree to 1e-5, integration continued elsewhere; timed
+         beside the plain method on the same sims, which also judges the error.
 
 Also reported: the error of the f32 results against the engine's FP64 path at
 rtol 1e-10 on a subset, the integrator kernel's achieved FP32 rate against the
@@ -184,12 +190,128 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
     }
 
 
+def run_closure(ce, args) -> dict:
+    import coulomb_explosion_amd.closure_train as ct
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    mol = ce.Molecule([ce.Atom("C", C_, 1.0)] + [ce.Atom("H", H_, 1.0)] * 4)
+    cfg = run_config(ce)
+    log(f"[bench] closure: generating {args.train_sims} training trajectories")
+    data = ct.make_data(ce, mol, args.train_sims, SEED + 1, cfg, args.t_cut, RADIUS, MIN_SEP,
+                        RTOL, ATOL)
+    log(f"[bench] closure: {len(data.final)} pairs in {data.seconds:.1f} s; training "
+        f"{args.members} members x {args.epochs} epochs")
+    ens = ct.fit(ce, data, members=args.members, hidden=(256, 256, 256), epochs=args.epochs,
+                 batch=args.batch, lr=args.lr, seed=1, log=log)
+    clo = ct.attach(ce, mol, ens)
+    n = args.eval_sims
+    eng = ce.BatchedRK45(mol, 1.0, n, RTOL, ATOL)
+    load_autotune(ce).apply_cached(ce, eng, cfg, key_extra="ch4")
+    eng.enable_state()
+    seed = SEED + 2
+
+    def plain():
+        eng.sample(seed, n, RADIUS, MIN_SEP)
+        eng.run(cfg, download=False)
+
+    def closure():
+        eng.sample(seed, n, RADIUS, MIN_SEP)
+        clo.run(eng, cfg, args.t_cut, args.gate_tol, download=False)
+
+    for _ in range(args.warmup):
+        plain()
+        closure()
+    times = {}
+    for name, fn in (("plain", plain), ("closure", closure)):
+        log(f"[bench] closure: timing {name} x {args.steps}")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        times[name] = (time.perf_counter() - t0) / args.steps
+        if name == "plain":
+            ref = eng.download()
+            plain_kernel_ms = eng.stats()["kernel_ms"]
+    res = eng.download()
+    st = clo.stats()
+    p_ref = ref.momenta.reshape(n, 5, 3)
+    err = (np.linalg.norm(res.momenta.reshape(n, 5, 3) - p_ref, axis=-1) /
+           np.linalg.norm(p_ref, axis=-1)).max(-1)
+    acc = res.status == ce.SIM_CLOSURE
+    ok = (err <= 1e-5) & (res.converged == 1)
+    # held-out quality of the network itself, gate aside
+    test = ct.make_data(ce, mol, 20000, SEED + 3, cfg, args.t_cut, RADIUS, MIN_SEP, RTOL, ATOL)
+    mean, spread = ct.predict(ens, test.features, test.prior)
+
+    def rel(p):
+        return (np.linalg.norm(p - test.final, axis=-1) / np.linalg.norm(test.final, axis=-1)).max(-1)
+
+    cur = test.features.reshape(len(test.final), -1)[:, :30].reshape(-1, 5, 6)[:, :, 3:]
+    quality = {"held_out_sims": len(test.final),
+               "median_err_current_momenta": float(np.median(rel(cur))),
+               "median_err_prior": float(np.median(rel(test.prior))),
+               "median_err_prior_plus_network": float(np.median(rel(mean))),
+               "p90_err_prior_plus_network": float(np.quantile(rel(mean), 0.9)),
+               "median_ensemble_spread": float(np.median(spread))}
+    mlp_flops = 2 * sum(a * b for a, b in zip([40, 256, 256, 256], [256, 256, 256, 15]))
+    infer_flops = st["candidates"] * mlp_flops * args.members
+    return {
+        "n": n,
+        "closure_s": times["closure"], "plain_s": times["plain"],
+        "within": int(ok.sum()),
+        "line_config": {
+            "workload": (f"CH4 learned closure, {n} fresh sims per step (R 2.5, min sep 1.0, "
+                         f"rtol 1e-6, atol 1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4): "
+                         f"plain DP5(4) to t_cut {args.t_cut:g}, straight-line impulse prior + "
+                         f"{args.members}-member MLP ensemble residual, gate {args.gate_tol:g} on "
+                         f"the members' per-atom spread, integration continued elsewhere"),
+            "sims_per_step": n,
+            "method": "closure",
+            "plain_sims_per_s": n / times["plain"],
+            "closure_sims_per_s_all": n / times["closure"],
+            "closure_over_plain": times["plain"] / times["closure"],
+            "fallback_rate": st["fallback"] / max(1, st["candidates"]),
+            "candidates": st["candidates"],
+            "accepted_by_gate": int(acc.sum()),
+            "err_vs_full_integration": {
+                "within_1e-5": float(np.mean(ok)),
+                "accepted_max": float(err[acc].max()) if acc.any() else None,
+                "accepted_median": float(np.median(err[acc])) if acc.any() else None,
+                "all_max": float(err.max())},
+            "phase_ms": {"cut": st["cut_ms"], "features_ensemble_gate": st["infer_ms"],
+                         "fallback": st["fallback_ms"], "total_device": st["total_ms"],
+                         "plain_kernel": plain_kernel_ms},
+            "attempts_per_sim": {"cut": st["cut_attempts"] / n,
+                                 "fallback": st["fallback_attempts"] / n},
+            "network_quality": quality,
+            "training": {"trajectories": len(data.final), "generate_s": data.seconds,
+                         "train_s": ens.seconds, "members": args.members,
+                         "widths": [40, 256, 256, 256, 15], "epochs": args.epochs,
+                         "batch": args.batch, "lr": args.lr,
+                         "final_loss": [float(h[-1]) for h in ens.losses]},
+        },
+        "roofline": {
+            "bound": "mfma", "kernel": "gemm_bf16 (ensemble forward, with features and gate)",
+            "achieved": infer_flops / (st["infer_ms"] / 1e3) / 1e12 if st["infer_ms"] else 0.0,
+            "peak": 2500.0, "unit": "TFLOP/s",
+            "frac": (infer_flops / (st["infer_ms"] / 1e3) / 1e12 / 2500.0) if st["infer_ms"] else 0.0,
+            "traffic": None,
+            "flops_per_sim_per_member": mlp_flops},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["ch4", "mixed", "n10"], default="ch4")
+    ap.add_argument("--workload", choices=["ch4", "mixed", "n10", "closure"], default="ch4")
+    ap.add_argument("--train-sims", type=int, default=2_000_000)
+    ap.add_argument("--eval-sims", type=int, default=8_000_000)
+    ap.add_argument("--members", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=8192)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--t-cut", type=float, default=200.0)
+    ap.add_argument("--gate-tol", type=float, default=1e-5)
     ap.add_argument("--sims", type=int, default=10_000_000)
     ap.add_argument("--accuracy-subset", type=int, default=100_000)
     ap.add_argument("--cpu-sims", type=int, default=400_000)
@@ -230,6 +352,17 @@ def main() -> int:
     if ce.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
+
+    if args.workload == "closure":
+        r = run_closure(ce, args)
+        line = {"metric": METRIC, "value": r["within"] / r["closure_s"], "unit": "sims/s",
+                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                "ms_per_step": r["closure_s"] * 1e3, "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "f32+bf16",
+                "data": f"synthetic (device sampler, train seed {SEED + 1}, eval seed {SEED + 2})",
+                "config": r["line_config"], "roofline": r["roofline"], "cpu_baseline": None}
+        print(json.dumps(line), flush=True)
+        return 0
 
     w = Workload(ce, args.workload, args.sims)
     cfg = run_config(ce)

@@ -38,7 +38,7 @@ PRECISION_F32 = 0
 PRECISION_F64 = 1
 
 SIM_OK_PE, SIM_OK_MOMENTUM, SIM_OK_TMAX, SIM_NOT_CONVERGED = 0, 1, 2, 3
-SIM_REJECT_LIMIT, SIM_NONFINITE, SIM_SAMPLER_FAIL = 4, 5, 6
+SIM_REJECT_LIMIT, SIM_NONFINITE, SIM_SAMPLER_FAIL, SIM_CLOSURE = 4, 5, 6, 7
 
 CE_OK, CE_ERR_INVALID_ARGUMENT, CE_ERR_RUNTIME, CE_ERR_HIP, CE_ERR_NO_DEVICE = 0, -1, -2, -3, -4
 
@@ -58,7 +58,7 @@ class _RunConfig(C.Structure):
                 ("momentum_tol", C.c_double), ("h0", C.c_double), ("max_steps", C.c_uint64),
                 ("stop_rule", C.c_int32), ("redistribute_energy", C.c_int32),
                 ("h0_mode", C.c_int32), ("max_rejections", C.c_int32), ("precision", C.c_int32),
-                ("pad_", C.c_int32)]
+                ("resume", C.c_int32)]
 
 
 class _Results(C.Structure):
@@ -82,6 +82,13 @@ class _SpeciesDesc(C.Structure):
                 ("masses", C.POINTER(C.c_double)), ("charges", C.POINTER(C.c_double)),
                 ("radius", C.c_double), ("min_separation", C.c_double), ("count", C.c_uint64),
                 ("max_total_charge", C.c_int32), ("pad_", C.c_int32)]
+
+
+class _ClosureStats(C.Structure):
+    _fields_ = [("sims", C.c_uint64), ("candidates", C.c_uint64), ("fallback", C.c_uint64),
+                ("cut_attempts", C.c_uint64), ("fallback_attempts", C.c_uint64),
+                ("cut_ms", C.c_double), ("infer_ms", C.c_double), ("fallback_ms", C.c_double),
+                ("total_ms", C.c_double)]
 
 
 class _LaunchConfig(C.Structure):
@@ -112,6 +119,9 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_engine_set_launch": ([P, P], C.c_int),
         "ce_engine_get_launch": ([P, P], C.c_int),
         "ce_engine_enable_trace": ([P, C.c_uint32], C.c_int),
+        "ce_engine_enable_state": ([P, C.c_int32], C.c_int),
+        "ce_engine_get_state": ([P, P, P, P, P, S], C.c_int),
+        "ce_engine_run_subset": ([P, P, P, S, P], C.c_int),
         "ce_engine_create_mixed": ([P, C.c_int32, C.c_double, P, P], C.c_int),
         "ce_engine_sample_species": ([P, C.c_uint64, C.c_uint64, C.c_uint32], C.c_int),
         "ce_engine_set_charges": ([P, P, S], C.c_int),
@@ -134,6 +144,13 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_mlp_adam_step": ([P, C.c_float], C.c_int),
         "ce_mlp_set_dataset": ([P, P, P, S], C.c_int),
         "ce_mlp_train_epoch": ([P, S, C.c_float, C.c_uint64, P], C.c_int),
+        "ce_closure_create": ([C.c_int32, P, P, C.c_double, P], C.c_int),
+        "ce_closure_destroy": ([P], C.c_int),
+        "ce_closure_set_normalization": ([P, P, P, P], C.c_int),
+        "ce_closure_set_members": ([P, P, C.c_int32], C.c_int),
+        "ce_closure_features": ([P, P, P, P, S], C.c_int),
+        "ce_closure_run": ([P, P, P, C.c_double, C.c_double, P], C.c_int),
+        "ce_closure_get_stats": ([P, P], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -207,11 +224,12 @@ class RunConfig:
     h0: float = 1e-3
     max_rejections: int = 100
     precision: int = PRECISION_F32
+    resume: bool = False  # continue from the saved state (BatchedRK45.enable_state)
 
     def _c(self) -> _RunConfig:
         return _RunConfig(self.pe_stop_fraction, self.max_dt, self.t_max, self.momentum_tol,
                           self.h0, self.max_steps, self.stop_rule, int(self.redistribute_energy),
-                          self.h0_mode, self.max_rejections, self.precision, 0)
+                          self.h0_mode, self.max_rejections, self.precision, int(self.resume))
 
 
 @dataclass
@@ -386,9 +404,31 @@ class BatchedRK45:
                        ptr(ee, C.c_double), ptr(tr, C.c_double) if tr is not None else None,
                        self._trace_cap, 0)
         _check(_lib.ce_engine_download(self._h, C.byref(res)))
-        conv = (st <= SIM_OK_TMAX).astype(np.uint8)
+        conv = ((st <= SIM_OK_TMAX) | (st == SIM_CLOSURE)).astype(np.uint8)
         return Result(mom.reshape(-1), steps, conv, int(steps.max()) if ns else 0, st, att, tf,
                       ee, mom32, tr)
+
+    def run_subset(self, sims, cfg: RunConfig | None = None) -> Result:
+        """Runs only the listed sims (indices into the loaded batch); the others
+        keep their previous results."""
+        cfg = cfg or RunConfig()
+        idx = np.ascontiguousarray(sims, dtype=np.uint64)
+        c = cfg._c()
+        _check(_lib.ce_engine_run_subset(self._h, C.byref(c), idx.ctypes.data, idx.size, None))
+        return self.download()
+
+    def enable_state(self, enable: bool = True) -> None:
+        """Keep every sim's integration state at its stop for a later resume."""
+        _check(_lib.ce_engine_enable_state(self._h, int(enable)))
+
+    def state(self) -> dict:
+        """The saved state: pos, vel [sims, n, 3], h and e0 per sim (float64)."""
+        ns = self._loaded
+        out = {"pos": np.zeros((ns, self.n, 3)), "vel": np.zeros((ns, self.n, 3)),
+               "h": np.zeros(ns), "e0": np.zeros(ns)}
+        _check(_lib.ce_engine_get_state(self._h, out["pos"].ctypes.data, out["vel"].ctypes.data,
+                                        out["h"].ctypes.data, out["e0"].ctypes.data, ns))
+        return out
 
     def records(self, first: int = 0, n_sims: int | None = None) -> np.ndarray:
         """Packed dataset records (examples/generate_dataset.cpp:198-217) as a
@@ -632,3 +672,62 @@ class MLP:
         loss = C.c_float(0.0)
         _check(_lib.ce_mlp_train_epoch(self._h, batch, lr, shuffle_seed, C.byref(loss)))
         return float(loss.value)
+
+
+class Closure:
+    """Learned asymptotic closure for one molecule (see include/coulomb_explosion.h):
+    plain DP5(4) to t_cut, straight-line-impulse prior plus an MLP ensemble's
+    residual where the members agree, integration continued elsewhere."""
+
+    def __init__(self, mol: Molecule, coulomb_k: float = 1.0):
+        self.n = mol.size()
+        self._m = np.ascontiguousarray(mol.masses)
+        self._q = np.ascontiguousarray(mol.charges)
+        h = C.c_void_p()
+        _check(_lib.ce_closure_create(self.n, self._m.ctypes.data, self._q.ctypes.data,
+                                      float(coulomb_k), C.byref(h)))
+        self._h = h
+        self._members = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.ce_closure_destroy(self._h)
+            self._h = None
+
+    @property
+    def n_features(self) -> int:
+        return 8 * self.n
+
+    def set_normalization(self, in_shift, in_scale, out_scale) -> None:
+        a = [np.ascontiguousarray(v, dtype=np.float32) for v in (in_shift, in_scale, out_scale)]
+        if a[0].size != 8 * self.n or a[1].size != 8 * self.n or a[2].size != 3 * self.n:
+            raise ValueError("normalisation sizes must be 8n, 8n, 3n")
+        _check(_lib.ce_closure_set_normalization(self._h, *[v.ctypes.data for v in a]))
+
+    def set_members(self, members) -> None:
+        arr = (C.c_void_p * max(1, len(members)))(*[m._h.value for m in members])
+        _check(_lib.ce_closure_set_members(self._h, arr, len(members)))
+        self._members = list(members)  # keep them alive
+
+    def features(self, eng: "BatchedRK45"):
+        """Raw features [sims, 8n] and prior momenta [sims, n, 3] of the engine's
+        saved state."""
+        ns = eng._loaded
+        f = np.zeros((ns, 8 * self.n), dtype=np.float32)
+        pr = np.zeros((ns, self.n, 3))
+        _check(_lib.ce_closure_features(self._h, eng._h, f.ctypes.data, pr.ctypes.data, ns))
+        return f, pr
+
+    def run(self, eng: "BatchedRK45", cfg: RunConfig, t_cut: float, gate_tol: float,
+            download: bool = True):
+        if eng._dirty:
+            _check(_lib.ce_engine_set_geometries(eng._h, eng._geom.ctypes.data, eng._loaded))
+            eng._dirty = False
+        c = cfg._c()
+        _check(_lib.ce_closure_run(self._h, eng._h, C.byref(c), t_cut, gate_tol, None))
+        return eng.download() if download else None
+
+    def stats(self) -> dict:
+        s = _ClosureStats()
+        _check(_lib.ce_closure_get_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in _ClosureStats._fields_}

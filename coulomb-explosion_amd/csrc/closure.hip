@@ -1,0 +1,521 @@
+// Learned asymptotic closure (no reference counterpart).
+//
+// A run integrates every sim with the plain DP5(4) up to an early cut time
+// t_cut (the engine's momentum-or-t_max rule with t_max = t_cut), keeping each
+// sim's state.  For the sims stopped by the cut, a feature kernel builds the
+// network input and a closed-form physical prior of the asymptotic momenta; an
+// ensemble of MLPs predicts the residual over the prior; a gate kernel accepts
+// the ensemble mean where the members agree to within the tolerance (per-atom
+// relative spread of the predicted momenta) and queues the rest, which the
+// engine then continues from their saved state to the full stop rule.
+//
+// Prior.  With every atom moving on a straight line from the cut on
+// (r_ij(tau) = r + u tau, u = v_i - v_j), the remaining Coulomb impulse of pair
+// (i, j) on atom i is
+//   J = k q_i q_j int_0^inf r(tau) / |r(tau)|^3 dtau
+//     = k q_i q_j (r/|r| + u/|u|) / (|r| |u| + r.u),
+// and the prior p_i + sum_j J_ij is rescaled so that its kinetic energy equals
+// E0 (all potential energy converted), as the engine's redistribution does.
+//
+// Features (8n per sim): for each atom its position relative to the centre of
+// mass and its momentum m v (6n), then the n charges and the n masses; each is
+// shifted and scaled by the normalisation set from the training data.
+// Outputs (3n): (p_final - p_prior) / out_scale per component.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "coulomb_explosion.h"
+#include "engine.hpp"
+#include "mlp.hpp"
+
+namespace {
+
+constexpr int kMaxMembers = 8;
+
+struct ClosureConst {
+  double m[CE_MAX_ATOMS];
+  double q[CE_MAX_ATOMS];
+  double k;
+};
+
+struct FeatureArgs {
+  const float* x;       // saved state [sim][n][3]
+  const float* v;
+  const double* e0;
+  const uint8_t* status;
+  const float* qsim;    // per-sim charges [sim][n] or nullptr
+  const float* shift;   // [8n] or nullptr (raw features)
+  const float* inv_scale;
+  uint64_t first, count;  // sims [first, first + count)
+  int all;                // 1: every sim, 0: only sims stopped by the cut
+  float* feat;            // [count][8n]
+  double* prior;          // [count][3n]
+};
+
+template <int N>
+__global__ void closure_features(const FeatureArgs a, const ClosureConst cc) {
+  const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (r >= a.count) return;
+  const uint64_t g = a.first + r;
+  if (!a.all && a.status[g] != ce::kSimOkTmax) return;
+  double x[3 * N], v[3 * N], q[N], p[3 * N];
+  double cm[3] = {0.0, 0.0, 0.0}, mt = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    q[i] = a.qsim != nullptr ? static_cast<double>(a.qsim[g * N + i]) : cc.q[i];
+    mt += cc.m[i];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      x[3 * i + c] = a.x[g * 3 * N + 3 * i + c];
+      v[3 * i + c] = a.v[g * 3 * N + 3 * i + c];
+      p[3 * i + c] = cc.m[i] * v[3 * i + c];
+      cm[c] += cc.m[i] * x[3 * i + c];
+    }
+  }
+  // prior: straight-line impulse of every pair, then KE rescaled to E0
+  double pp[3 * N];
+#pragma unroll
+  for (int j = 0; j < 3 * N; ++j) pp[j] = p[j];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int j = i + 1; j < N; ++j) {
+      double rr[3], uu[3];
+      double c2 = 0.0, a2 = 0.0, b = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rr[c] = x[3 * i + c] - x[3 * j + c];
+        uu[c] = v[3 * i + c] - v[3 * j + c];
+        c2 += rr[c] * rr[c];
+        a2 += uu[c] * uu[c];
+        b += rr[c] * uu[c];
+      }
+      const double rn = sqrt(c2), un = sqrt(a2);
+      const double w = rn * un + b;
+      if (!(rn > 0.0) || !(un > 0.0) || !(w > 1e-300)) continue;
+      const double s = cc.k * q[i] * q[j] / w;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double J = s * (rr[c] / rn + uu[c] / un);
+        pp[3 * i + c] += J;
+        pp[3 * j + c] -= J;
+      }
+    }
+  }
+  double ke = 0.0;
+#pragma unroll
+  for (int i = 0; i < N; ++i)
+    ke += 0.5 * (pp[3 * i] * pp[3 * i] + pp[3 * i + 1] * pp[3 * i + 1] +
+                 pp[3 * i + 2] * pp[3 * i + 2]) / cc.m[i];
+  const double sc = (ke > 0.0 && a.e0[g] > 0.0) ? sqrt(a.e0[g] / ke) : 1.0;
+  double* pr = a.prior + r * 3 * N;
+#pragma unroll
+  for (int j = 0; j < 3 * N; ++j) pr[j] = pp[j] * sc;
+  // features
+  float f[8 * N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      f[6 * i + c] = static_cast<float>(x[3 * i + c] - cm[c] / mt);
+      f[6 * i + 3 + c] = static_cast<float>(p[3 * i + c]);
+    }
+    f[6 * N + i] = static_cast<float>(q[i]);
+    f[7 * N + i] = static_cast<float>(cc.m[i]);
+  }
+  float* fo = a.feat + r * 8 * N;
+#pragma unroll
+  for (int j = 0; j < 8 * N; ++j)
+    fo[j] = a.shift != nullptr ? (f[j] - a.shift[j]) * a.inv_scale[j] : f[j];
+}
+
+struct GateArgs {
+  const double* prior;      // [count][3n]
+  const float* y;           // [members][count][3n]
+  const float* out_scale;   // [3n]
+  int members;
+  uint64_t first, count;
+  double tol2;              // gate on (spread / |p|)^2 per atom
+  uint8_t* status;          // global sim index
+  double* mom;              // [sim][n][3]
+  float* mom32;
+  uint32_t* list;           // fallback sims
+  uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut
+};
+
+template <int N>
+__global__ void closure_gate(const GateArgs a) {
+  const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  if (r >= a.count) return;
+  const uint64_t g = a.first + r;
+  if (a.status[g] != ce::kSimOkTmax) return;
+  atomicAdd(a.counts + 1, 1u);
+  const uint64_t cs = a.count * 3 * N;  // member stride
+  bool ok = a.members >= 2;
+  double pm[3 * N];
+#pragma unroll
+  for (int j = 0; j < 3 * N; ++j) {
+    double s = 0.0;
+    for (int k = 0; k < a.members; ++k) s += a.y[k * cs + r * 3 * N + j];
+    pm[j] = a.members > 0 ? s / a.members : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double var = 0.0, p2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int j = 3 * i + c;
+      for (int k = 0; k < a.members; ++k) {
+        const double d = (a.y[k * cs + r * 3 * N + j] - pm[j]) * a.out_scale[j];
+        var += d * d;
+      }
+      const double pj = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];
+      p2 += pj * pj;
+    }
+    var /= (a.members > 1 ? a.members - 1 : 1);
+    ok = ok && (var <= a.tol2 * p2);  // false for NaN
+  }
+  if (ok) {
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) {
+      const double pj = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];
+      a.mom[g * 3 * N + j] = pj;
+      a.mom32[g * 3 * N + j] = static_cast<float>(pj);
+    }
+    a.status[g] = ce::kSimClosure;
+  } else {
+    const uint32_t slot = atomicAdd(a.counts, 1u);
+    a.list[slot] = static_cast<uint32_t>(g);
+  }
+}
+
+__global__ void zero_counts(uint32_t* p) {
+  if (threadIdx.x < 2 && blockIdx.x == 0) p[threadIdx.x] = 0u;
+}
+
+}  // namespace
+
+struct ce_closure {
+  int n = 0;
+  ClosureConst cc{};
+  int device = 0;
+  std::vector<ce_mlp*> members;
+  float* shift = nullptr;      // [8n]
+  float* inv_scale = nullptr;  // [8n]
+  float* out_scale = nullptr;  // [3n]
+  bool normalized = false;
+  int64_t chunk_cap = 0;
+  float* feat = nullptr;
+  double* prior = nullptr;
+  float* y = nullptr;          // [kMaxMembers][chunk][3n]
+  uint32_t* counts = nullptr;  // fallback queued, cut-stopped sims
+  hipEvent_t ev[4] = {};
+  ce_closure_stats stats{};
+};
+
+namespace {
+
+int fail(int code, const std::string& m) { return ce::set_error(code, m); }
+
+#define CL_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+int dalloc(T** p, int64_t n) {
+  *p = nullptr;
+  CL_HIP(hipMalloc(reinterpret_cast<void**>(p), (n > 0 ? n : 1) * sizeof(T)));
+  return CE_OK;
+}
+
+void free_closure(ce_closure* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  void* b[] = {c->shift, c->inv_scale, c->out_scale, c->feat, c->prior, c->y, c->counts};
+  for (void* p : b)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  delete c;
+}
+
+int ensure_chunk(ce_closure* c, int64_t n) {
+  if (n <= c->chunk_cap) return CE_OK;
+  void* b[] = {c->feat, c->prior, c->y};
+  for (void* p : b)
+    if (p) CL_HIP(hipFree(p));
+  c->feat = nullptr;
+  c->prior = nullptr;
+  c->y = nullptr;
+  c->chunk_cap = 0;
+  int rc = dalloc(&c->feat, n * 8 * c->n);
+  if (rc == CE_OK) rc = dalloc(&c->prior, n * 3 * c->n);
+  if (rc == CE_OK) rc = dalloc(&c->y, int64_t(kMaxMembers) * n * 3 * c->n);
+  if (rc == CE_OK) c->chunk_cap = n;
+  return rc;
+}
+
+unsigned blocks(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+
+int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
+  switch (c->n) {
+#define CL_F(NN)                                                                      \
+  case NN:                                                                            \
+    hipLaunchKernelGGL(closure_features<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a, \
+                       c->cc);                                                        \
+    break;
+    CL_F(1) CL_F(2) CL_F(3) CL_F(4) CL_F(5) CL_F(6) CL_F(7) CL_F(8) CL_F(9) CL_F(10)
+#undef CL_F
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CL_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+int launch_gate(ce_closure* c, const GateArgs& a, hipStream_t s) {
+  switch (c->n) {
+#define CL_G(NN)                                                                          \
+  case NN:                                                                                \
+    hipLaunchKernelGGL(closure_gate<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a); \
+    break;
+    CL_G(1) CL_G(2) CL_G(3) CL_G(4) CL_G(5) CL_G(6) CL_G(7) CL_G(8) CL_G(9) CL_G(10)
+#undef CL_G
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CL_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+int check_engine(const ce_closure* c, const ce_engine* e) {
+  if (e->n_species != 1 || e->sp[0].n != c->n)
+    return fail(CE_ERR_INVALID_ARGUMENT, "the closure needs a single-species engine of its molecule size");
+  return CE_OK;
+}
+
+FeatureArgs feature_args(const ce_closure* c, const ce_engine* e, uint64_t first, uint64_t count,
+                         int all, bool normalize) {
+  FeatureArgs a{};
+  a.x = static_cast<const float*>(e->state_x);
+  a.v = static_cast<const float*>(e->state_v);
+  a.e0 = e->state_e0;
+  a.status = e->status;
+  a.qsim = e->have_charges ? e->qsim : nullptr;
+  a.shift = normalize ? c->shift : nullptr;
+  a.inv_scale = normalize ? c->inv_scale : nullptr;
+  a.first = first;
+  a.count = count;
+  a.all = all;
+  a.feat = c->feat;
+  a.prior = c->prior;
+  return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ce_closure_create(int32_t n_atoms, const double* masses, const double* charges,
+                      double coulomb_k, ce_closure** out) {
+  if (out == nullptr || masses == nullptr || charges == nullptr)
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (n_atoms < 2 || n_atoms > CE_MAX_ATOMS)
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms must be in [2, CE_MAX_ATOMS]");
+  for (int i = 0; i < n_atoms; ++i)
+    if (!(masses[i] > 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "masses must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CE_ERR_NO_DEVICE, "no HIP device");
+  ce_closure* c = new ce_closure();
+  c->n = n_atoms;
+  for (int i = 0; i < n_atoms; ++i) {
+    c->cc.m[i] = masses[i];
+    c->cc.q[i] = charges[i];
+  }
+  c->cc.k = coulomb_k;
+  int rc = CE_OK;
+  if (hipGetDevice(&c->device) != hipSuccess) rc = fail(CE_ERR_HIP, "hipGetDevice");
+  if (rc == CE_OK) rc = dalloc(&c->shift, 8 * n_atoms);
+  if (rc == CE_OK) rc = dalloc(&c->inv_scale, 8 * n_atoms);
+  if (rc == CE_OK) rc = dalloc(&c->out_scale, 3 * n_atoms);
+  if (rc == CE_OK) rc = dalloc(&c->counts, 2);
+  for (int i = 0; i < 4 && rc == CE_OK; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CE_ERR_HIP, "hipEventCreate");
+  if (rc != CE_OK) {
+    free_closure(c);
+    return rc;
+  }
+  *out = c;
+  return CE_OK;
+}
+
+int ce_closure_destroy(ce_closure* c) {
+  free_closure(c);
+  return CE_OK;
+}
+
+int ce_closure_set_normalization(ce_closure* c, const float* in_shift, const float* in_scale,
+                                 const float* out_scale) {
+  if (!c || !in_shift || !in_scale || !out_scale) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  const int f = 8 * c->n, o = 3 * c->n;
+  std::vector<float> inv(f);
+  for (int j = 0; j < f; ++j) {
+    if (!(in_scale[j] > 0.0f)) return fail(CE_ERR_INVALID_ARGUMENT, "input scales must be positive");
+    inv[j] = 1.0f / in_scale[j];
+  }
+  for (int j = 0; j < o; ++j)
+    if (!(out_scale[j] > 0.0f)) return fail(CE_ERR_INVALID_ARGUMENT, "output scales must be positive");
+  CL_HIP(hipSetDevice(c->device));
+  CL_HIP(hipMemcpy(c->shift, in_shift, f * sizeof(float), hipMemcpyHostToDevice));
+  CL_HIP(hipMemcpy(c->inv_scale, inv.data(), f * sizeof(float), hipMemcpyHostToDevice));
+  CL_HIP(hipMemcpy(c->out_scale, out_scale, o * sizeof(float), hipMemcpyHostToDevice));
+  c->normalized = true;
+  return CE_OK;
+}
+
+int ce_closure_set_members(ce_closure* c, ce_mlp* const* members, int32_t n_members) {
+  if (!c || (!members && n_members > 0)) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (n_members < 0 || n_members > kMaxMembers)
+    return fail(CE_ERR_INVALID_ARGUMENT, "0..8 ensemble members");
+  for (int k = 0; k < n_members; ++k) {
+    const ce_mlp* m = members[k];
+    if (m == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL member");
+    if (ce::mlp_width(m, 0) != 8 * c->n || ce::mlp_width(m, ce::mlp_layers(m)) != 3 * c->n)
+      return fail(CE_ERR_INVALID_ARGUMENT, "members must map 8n features to 3n outputs");
+  }
+  c->members.assign(members, members + n_members);
+  return CE_OK;
+}
+
+int ce_closure_features(ce_closure* c, ce_engine* e, float* features, double* prior,
+                        size_t n_sims) {
+  if (!c || !e) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_engine(c, e);
+  if (rc != CE_OK) return rc;
+  if (e->state_prec != CE_PRECISION_F32)
+    return fail(CE_ERR_RUNTIME, "no f32 saved state: enable the state and run the engine first");
+  if (n_sims != e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims must equal the loaded sims");
+  CL_HIP(hipSetDevice(c->device));
+  const int64_t chunk = 1 << 20;
+  rc = ensure_chunk(c, std::min<int64_t>(chunk, std::max<int64_t>(1, n_sims)));
+  if (rc != CE_OK) return rc;
+  for (uint64_t f0 = 0; f0 < n_sims; f0 += chunk) {
+    const uint64_t cnt = std::min<uint64_t>(chunk, n_sims - f0);
+    rc = launch_features(c, feature_args(c, e, f0, cnt, 1, false), e->stream);
+    if (rc != CE_OK) return rc;
+    if (features)
+      CL_HIP(hipMemcpyAsync(features + f0 * 8 * c->n, c->feat, cnt * 8 * c->n * sizeof(float),
+                            hipMemcpyDeviceToHost, e->stream));
+    if (prior)
+      CL_HIP(hipMemcpyAsync(prior + f0 * 3 * c->n, c->prior, cnt * 3 * c->n * sizeof(double),
+                            hipMemcpyDeviceToHost, e->stream));
+    CL_HIP(hipStreamSynchronize(e->stream));
+  }
+  return CE_OK;
+}
+
+// Integrates to t_cut, applies the closure where the ensemble agrees and
+// continues the other sims to the stop rule of `cfg`.
+int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double t_cut,
+                   double gate_tol, ce_results* out) {
+  if (!c || !e || !cfg) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  int rc = check_engine(c, e);
+  if (rc != CE_OK) return rc;
+  if (cfg->stop_rule != CE_STOP_MOMENTUM_OR_TMAX || cfg->precision != CE_PRECISION_F32 || cfg->resume)
+    return fail(CE_ERR_INVALID_ARGUMENT, "the closure runs the f32 momentum-or-t_max rule");
+  if (!(t_cut > 0.0) || !(t_cut < cfg->t_max))
+    return fail(CE_ERR_INVALID_ARGUMENT, "t_cut must be in (0, t_max)");
+  if (!(gate_tol >= 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "gate_tol must be >= 0");
+  if (!c->members.empty() && !c->normalized)
+    return fail(CE_ERR_RUNTIME, "set the normalisation before running members");
+  CL_HIP(hipSetDevice(c->device));
+  rc = ce_engine_enable_state(e, 1);
+  if (rc != CE_OK) return rc;
+  if (e->list == nullptr) {
+    CL_HIP(hipMalloc(reinterpret_cast<void**>(&e->list), std::max<size_t>(1, e->capacity) * sizeof(uint32_t)));
+  }
+  ce_closure_stats st{};
+  const uint64_t n = e->loaded();
+  st.sims = n;
+  hipStream_t s = e->stream;
+  CL_HIP(hipEventRecord(c->ev[0], s));
+  // 1. plain integration to the cut
+  ce_run_config cut = *cfg;
+  cut.t_max = t_cut;
+  rc = ce::engine_run(e, &cut, nullptr, 0);
+  if (rc != CE_OK) return rc;
+  st.cut_ms = e->stats.kernel_ms;
+  st.cut_attempts = e->stats.attempted_steps;
+  // 2. features, ensemble and gate, chunk by chunk
+  CL_HIP(hipEventRecord(c->ev[1], s));
+  hipLaunchKernelGGL(zero_counts, dim3(1), dim3(64), 0, s, c->counts);
+  const int M = static_cast<int>(c->members.size());
+  const int64_t chunk = 1 << 20;
+  rc = ensure_chunk(c, std::min<int64_t>(chunk, std::max<int64_t>(1, n)));
+  if (rc != CE_OK) return rc;
+  const int o = 3 * c->n;
+  for (uint64_t f0 = 0; f0 < n && rc == CE_OK; f0 += chunk) {
+    const uint64_t cnt = std::min<uint64_t>(chunk, n - f0);
+    rc = launch_features(c, feature_args(c, e, f0, cnt, 0, true), s);
+    for (int k = 0; k < M && rc == CE_OK; ++k)
+      rc = ce::mlp_forward_device(c->members[k], c->feat, static_cast<int64_t>(cnt),
+                                  c->y + static_cast<int64_t>(k) * cnt * o, s);
+    if (rc != CE_OK) break;
+    GateArgs g{};
+    g.prior = c->prior;
+    g.y = c->y;
+    g.out_scale = c->out_scale;
+    g.members = M;
+    g.first = f0;
+    g.count = cnt;
+    g.tol2 = gate_tol * gate_tol;
+    g.status = e->status;
+    g.mom = e->mom;
+    g.mom32 = e->mom32;
+    g.list = e->list;
+    g.counts = c->counts;
+    rc = launch_gate(c, g, s);
+  }
+  if (rc != CE_OK) return rc;
+  uint32_t counts[2] = {0, 0};
+  CL_HIP(hipMemcpyAsync(counts, c->counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+  CL_HIP(hipEventRecord(c->ev[2], s));
+  CL_HIP(hipStreamSynchronize(s));
+  const uint32_t n_fb = counts[0];
+  st.fallback = n_fb;
+  st.candidates = counts[1];
+  // 3. fallback: continue the queued sims from their saved state
+  if (n_fb > 0) {
+    ce_run_config full = *cfg;
+    full.resume = 1;
+    rc = ce::engine_run(e, &full, e->list, n_fb);
+    if (rc != CE_OK) return rc;
+    st.fallback_ms = e->stats.kernel_ms;
+    st.fallback_attempts = e->stats.attempted_steps;
+  }
+  CL_HIP(hipEventRecord(c->ev[3], s));
+  CL_HIP(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  CL_HIP(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+  st.infer_ms = ms;
+  CL_HIP(hipEventElapsedTime(&ms, c->ev[0], c->ev[3]));
+  st.total_ms = ms;
+  e->stats.sims = n;
+  c->stats = st;
+  return out != nullptr ? ce_engine_download(e, out) : CE_OK;
+}
+
+int ce_closure_get_stats(const ce_closure* c, ce_closure_stats* out) {
+  if (!c || !out) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = c->stats;
+  return CE_OK;
+}
+
+}  // extern "C"

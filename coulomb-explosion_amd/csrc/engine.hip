@@ -17,6 +17,7 @@
 
 #include "coulomb_explosion.h"
 #include "dispatch.hpp"
+#include "engine.hpp"
 
 namespace {
 
@@ -42,85 +43,21 @@ int hip_alloc(T** p, size_t count) {
   return CE_OK;
 }
 
-struct Species {
-  int n = 0;
-  double masses[CE_MAX_ATOMS] = {};
-  double charges[CE_MAX_ATOMS] = {};
-  double radius = 0, min_sep = 0;
-  int levels = 0;      // per-sim charge levels (0 = fixed charges)
-  int max_total = 0;
-  uint64_t capacity = 0;  // sims this species can hold
-  uint64_t count = 0;     // sims loaded for the next run
-};
-
 }  // namespace
 
 namespace ce {
 int set_error(int code, const std::string& msg) { return fail(code, msg); }
 }  // namespace ce
 
-struct ce_engine {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  hipEvent_t k0 = nullptr, k1 = nullptr, s0 = nullptr, s1 = nullptr, p0 = nullptr, p1 = nullptr;
-  int n_species = 0;
-  Species sp[CE_MAX_SPECIES];
-  bool mixed = false;        // created by ce_engine_create_mixed (fixed species counts)
-  double coulomb_k = 1.0;
-  double rtol = 0, atol = 0;
-  size_t total_pos = 0;      // sum capacity * 3n (doubles)
-  size_t total_q = 0;        // sum capacity * n
-  size_t capacity = 0;       // sum of species capacities
-  bool sampled = false;
-  bool have_charges = false; // qsim holds per-sim charges for the loaded sims
-  // device buffers
-  double* geom = nullptr;
-  uint8_t* gstat = nullptr;
-  float* qsim = nullptr;
-  unsigned long long* ctr = nullptr;
-  double* mom = nullptr;
-  float* mom32 = nullptr;
-  uint32_t* steps = nullptr;
-  uint32_t* attempts = nullptr;
-  uint8_t* status = nullptr;
-  double* tfin = nullptr;
-  double* eerr = nullptr;
-  double* trace = nullptr;
-  uint32_t trace_cap = 0;
-  uint8_t* records = nullptr;
-  size_t records_cap = 0;
-  // launch
-  ce_launch_config launch = {0, 0, 4, 64};
-  int n_cus = 0;
-  ce_run_stats stats = {};
-  bool have_results = false;
-
-  size_t loaded() const {
-    size_t t = 0;
-    for (int s = 0; s < n_species; ++s) t += sp[s].count;
-    return t;
-  }
-  // start / offsets of species s for the loaded counts (single species: 0)
-  void offsets(int s, uint64_t* start, uint64_t* pos_off, uint64_t* q_off) const {
-    uint64_t a = 0, b = 0, c = 0;
-    for (int r = 0; r < s; ++r) {
-      a += sp[r].capacity;
-      b += sp[r].capacity * 3 * sp[r].n;
-      c += sp[r].capacity * sp[r].n;
-    }
-    *start = a;
-    *pos_off = b;
-    *q_off = c;
-  }
-};
 
 namespace {
 
 void free_engine(ce_engine* e) {
   if (e == nullptr) return;
   (void)hipSetDevice(e->device);
-  void* bufs[] = {e->geom,   e->gstat,    e->qsim,   e->ctr,  e->mom,  e->mom32,  e->steps,
-                  e->attempts, e->status, e->tfin, e->eerr, e->trace, e->records};
+  void* bufs[] = {e->geom,   e->gstat,    e->qsim,   e->ctr,  e->mom,     e->mom32,
+                  e->steps,  e->attempts, e->status, e->tfin, e->eerr,    e->trace,
+                  e->records, e->state_x, e->state_v, e->state_h, e->state_e0, e->list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   hipEvent_t evs[] = {e->k0, e->k1, e->s0, e->s1, e->p0, e->p1};
@@ -382,7 +319,7 @@ int ce_engine_create(const ce_molecule_desc* mol, const ce_tolerances* tol, size
     return fail(CE_ERR_NO_DEVICE, "no HIP device");
   ce_engine* e = new ce_engine();
   e->n_species = 1;
-  Species& s = e->sp[0];
+  ce::Species& s = e->sp[0];
   s.n = mol->n_atoms;
   for (int i = 0; i < s.n; ++i) {
     s.masses[i] = mol->masses[i];
@@ -423,7 +360,7 @@ int ce_engine_create_mixed(const ce_species_desc* species, int32_t n_species, do
   e->coulomb_k = coulomb_k;
   for (int s = 0; s < n_species; ++s) {
     const ce_species_desc& d = species[s];
-    Species& sp = e->sp[s];
+    ce::Species& sp = e->sp[s];
     sp.n = d.n_atoms;
     for (int i = 0; i < sp.n; ++i) {
       sp.masses[i] = d.masses[i];
@@ -531,7 +468,7 @@ int ce_engine_sample_species(ce_engine* e, uint64_t seed, uint64_t first_index,
   for (int s = 0; s < e->n_species; ++s) {
     uint64_t start, pos_off, q_off;
     e->offsets(s, &start, &pos_off, &q_off);
-    const Species& sp = e->sp[s];
+    const ce::Species& sp = e->sp[s];
     const int rc = sample_segment(e, s, seed, first_index + start, sp.radius, sp.min_sep,
                                   max_attempts, sp.levels > 0);
     if (rc != CE_OK) return rc;
@@ -540,7 +477,7 @@ int ce_engine_sample_species(ce_engine* e, uint64_t seed, uint64_t first_index,
   if (any_levels) {
     // species with fixed charges still need their charges in the per-sim table
     for (int s = 0; s < e->n_species; ++s) {
-      const Species& sp = e->sp[s];
+      const ce::Species& sp = e->sp[s];
       if (sp.levels > 0) continue;
       uint64_t start, pos_off, q_off;
       e->offsets(s, &start, &pos_off, &q_off);
@@ -604,8 +541,103 @@ int ce_engine_get_launch(const ce_engine* e, ce_launch_config* lc) {
 
 int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  const int rc = ce::engine_run(e, cfg, nullptr, 0);
+  return rc != CE_OK ? rc : copy_results(e, out);
+}
+
+int ce_engine_run_subset(ce_engine* e, const ce_run_config* cfg, const uint64_t* sims,
+                         size_t n, ce_results* out) {
+  if (e == nullptr || (sims == nullptr && n > 0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (e->n_species != 1) return fail(CE_ERR_INVALID_ARGUMENT, "subsets need a single-species engine");
+  if (n > e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "more sims than loaded");
+  std::vector<uint32_t> idx(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (sims[i] >= e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "sim index out of range");
+    idx[i] = static_cast<uint32_t>(sims[i]);
+  }
+  CE_HIP(hipSetDevice(e->device));
+  if (e->list == nullptr) {
+    const int arc = hip_alloc(&e->list, e->capacity);
+    if (arc != CE_OK) return arc;
+  }
+  if (n > 0) {
+    CE_HIP(hipMemcpyAsync(e->list, idx.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                          e->stream));
+    CE_HIP(hipStreamSynchronize(e->stream));
+  }
+  const int rc = ce::engine_run(e, cfg, e->list, n);
+  return rc != CE_OK ? rc : copy_results(e, out);
+}
+
+int ce_engine_enable_state(ce_engine* e, int32_t enable) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  CE_HIP(hipSetDevice(e->device));
+  CE_HIP(hipStreamSynchronize(e->stream));
+  if (!enable) {
+    void* b[] = {e->state_x, e->state_v, e->state_h, e->state_e0};
+    for (void* p : b)
+      if (p) CE_HIP(hipFree(p));
+    e->state_x = e->state_v = nullptr;
+    e->state_h = e->state_e0 = nullptr;
+    e->state_prec = -1;
+    return CE_OK;
+  }
+  if (e->state_x != nullptr) return CE_OK;
+  double *x = nullptr, *v = nullptr;
+  int rc = hip_alloc(&x, e->total_pos);  // room for T = double
+  if (rc == CE_OK) rc = hip_alloc(&v, e->total_pos);
+  if (rc == CE_OK) rc = hip_alloc(&e->state_h, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e->state_e0, e->capacity);
+  e->state_x = x;
+  e->state_v = v;
+  e->state_prec = -1;
+  return rc;
+}
+
+int ce_engine_get_state(ce_engine* e, double* pos, double* vel, double* h, double* e0,
+                        size_t n_sims) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (e->state_prec < 0) return fail(CE_ERR_RUNTIME, "no saved state: enable it and run first");
+  if (n_sims != e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims must equal the loaded sims");
+  CE_HIP(hipSetDevice(e->device));
+  size_t n_pos = 0;
+  for (int s = 0; s < e->n_species; ++s) n_pos += e->sp[s].count * 3 * e->sp[s].n;
+  const bool f32 = e->state_prec == CE_PRECISION_F32;
+  const size_t el = f32 ? sizeof(float) : sizeof(double);
+  double* dst[2] = {pos, vel};
+  void* src[2] = {e->state_x, e->state_v};
+  for (int b = 0; b < 2; ++b) {
+    if (dst[b] == nullptr) continue;
+    std::vector<char> raw(n_pos * el);
+    int rc = copy_per_atom(e, raw.data(), src[b], el, 3, hipMemcpyDeviceToHost);
+    if (rc != CE_OK) return rc;
+    CE_HIP(hipStreamSynchronize(e->stream));
+    for (size_t i = 0; i < n_pos; ++i)
+      dst[b][i] = f32 ? static_cast<double>(reinterpret_cast<const float*>(raw.data())[i])
+                      : reinterpret_cast<const double*>(raw.data())[i];
+  }
+  int rc = CE_OK;
+  if (h) rc = copy_per_sim(e, h, e->state_h, sizeof(double), hipMemcpyDeviceToHost);
+  if (rc == CE_OK && e0) rc = copy_per_sim(e, e0, e->state_e0, sizeof(double), hipMemcpyDeviceToHost);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipStreamSynchronize(e->stream));
+  return CE_OK;
+}
+
+}  // extern "C"
+
+namespace ce {
+
+// Runs the loaded sims (or the segment-local sims of the device list `list`,
+// single-species engines) with `cfg`; results stay on the device.
+int engine_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, size_t n_list) {
   int rc = check_config(cfg);
   if (rc != CE_OK) return rc;
+  if (cfg->resume && (e->state_x == nullptr || e->state_prec != cfg->precision))
+    return fail(CE_ERR_RUNTIME, "resume needs a saved state from a run of the same precision");
+  if (list != nullptr && e->n_species != 1)
+    return fail(CE_ERR_INVALID_ARGUMENT, "sim lists need a single-species engine");
   CE_HIP(hipSetDevice(e->device));
   ce::LaunchReq req{};
   ce::RunArgs& a = req.args;
@@ -639,11 +671,17 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   a.chunk = std::max(64, e->launch.chunk);
   a.n_segments = e->n_species;
   a.queue_slot = 0;
+  a.state_x = e->state_x;
+  a.state_v = e->state_v;
+  a.state_h = e->state_h;
+  a.state_e0 = e->state_e0;
+  a.resume = cfg->resume ? 1 : 0;
+  a.sim_list = list;
   for (int s = 0; s < e->n_species; ++s) {
     uint64_t start, pos_off, q_off;
     e->offsets(s, &start, &pos_off, &q_off);
-    a.seg[s] = ce::Segment{start, e->sp[s].count, pos_off, q_off, e->sp[s].n,
-                           e->have_charges ? 1 : 0};
+    a.seg[s] = ce::Segment{start, list != nullptr ? n_list : e->sp[s].count, pos_off, q_off,
+                           e->sp[s].n, e->have_charges ? 1 : 0};
     req.species[s].n = e->sp[s].n;
     for (int i = 0; i < e->sp[s].n; ++i) {
       req.species[s].masses[i] = e->sp[s].masses[i];
@@ -663,7 +701,8 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   CE_HIP(hipEventRecord(e->k0, e->stream));
   ce::LaunchInfo info;
   bool bad_n = false;
-  if (e->loaded() > 0) {
+  const size_t n_run = list != nullptr ? n_list : e->loaded();
+  if (n_run > 0) {
     const hipError_t he = (cfg->precision == CE_PRECISION_F64)
                               ? ce::launch_dp5_f64(req, &info, &bad_n)
                               : ce::launch_dp5_f32(req, &info, &bad_n);
@@ -677,7 +716,7 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   float ms = 0.0f;
   CE_HIP(hipEventElapsedTime(&ms, e->k0, e->k1));
   e->stats.kernel_ms = ms;
-  e->stats.sims = e->loaded();
+  e->stats.sims = n_run;
   e->stats.attempted_steps = host_ctr[0];
   e->stats.accepted_steps = host_ctr[1];
   e->stats.refills = host_ctr[2];
@@ -686,8 +725,13 @@ int ce_engine_run(ce_engine* e, const ce_run_config* cfg, ce_results* out) {
   e->stats.n_cus = e->n_cus;
   e->stats.launches = info.launches;
   e->have_results = true;
-  return copy_results(e, out);
+  if (e->state_x != nullptr) e->state_prec = cfg->precision;
+  return CE_OK;
 }
+
+}  // namespace ce
+
+extern "C" {
 
 int ce_engine_download(ce_engine* e, ce_results* out) {
   if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");

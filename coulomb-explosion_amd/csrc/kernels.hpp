@@ -267,6 +267,7 @@ enum : int {
   kSimRejectLimit = 4,
   kSimNonfinite = 5,
   kSimSamplerFail = 6,
+  kSimClosure = 7,
   kRunning = 255,
 };
 
@@ -310,6 +311,19 @@ struct RunArgs {
   int32_t n_segments;
   int32_t queue_slot;         // single-size launches: which queue head seg[0] uses
   Segment seg[kMaxSegments];
+  // Integration state at the stop (positions and velocities as T [sim][n][3] at
+  // pos_off, next step size and E0 per global sim), written when state_x is set;
+  // with resume = 1 every sim starts from it (and from its steps, attempts and
+  // t_final) instead of from geom.
+  void* state_x;
+  void* state_v;
+  double* state_h;
+  double* state_e0;
+  int32_t resume;
+  int32_t pad_;
+  // Single-segment launches: queue entry q integrates segment-local sim
+  // sim_list[q] (seg[0].count entries), or sim q when nullptr.
+  const uint32_t* sim_list;
 };
 
 struct Totals {
@@ -379,7 +393,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         assigned += take;
       }
       if (!active && mine != ~0ull) {
-        loc = mine;
+        loc = p.sim_list != nullptr ? static_cast<uint64_t>(p.sim_list[mine]) : mine;
         active = true;
         steps = 0;
         att = 0;
@@ -387,11 +401,21 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         rej_once = false;
         t = 0.0;
         status = kRunning;
-        const double* g = p.geom + sg.pos_off + loc * N3;
+        if (p.resume) {
+          const T* sx = static_cast<const T*>(p.state_x) + sg.pos_off + loc * N3;
+          const T* sv = static_cast<const T*>(p.state_v) + sg.pos_off + loc * N3;
 #pragma unroll
-        for (int q = 0; q < N3; ++q) {
-          x[q] = static_cast<T>(g[q]);
-          v[q] = T(0);
+          for (int q = 0; q < N3; ++q) {
+            x[q] = sx[q];
+            v[q] = sv[q];
+          }
+        } else {
+          const double* g = p.geom + sg.pos_off + loc * N3;
+#pragma unroll
+          for (int q = 0; q < N3; ++q) {
+            x[q] = static_cast<T>(g[q]);
+            v[q] = T(0);
+          }
         }
         if constexpr (PS) {
           const float* qs = p.qsim + sg.q_off + loc * N;
@@ -401,7 +425,19 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
             pk.kq[i] = static_cast<T>(p.coulomb_k * static_cast<double>(qs[i]));
           }
         }
-        if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
+        if (p.resume) {
+          // Continue a stopped integration: FSAL force at the saved positions.
+          const uint64_t gs = sg.start + loc;
+          steps = p.steps[gs];
+          att = p.attempts[gs];
+          t = p.t_final[gs];
+          h = static_cast<T>(p.state_h[gs]);
+          e0 = p.state_e0[gs];
+          pe_stop = p.pe_stop_fraction * e0;
+          T unused;
+          lane_force<T, N, PS, false, false>(x, k[0], unused, pk);
+          if (steps >= p.max_steps) status = kSimNotConverged;
+        } else if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
           status = kSimSamplerFail;
           e0 = 0.0;
         } else {
@@ -517,14 +553,16 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
         T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
-        h = minT(hh * f, cap);
+        const bool final_step = by_time && hh >= cap;
+        // After the step that lands on t_max the proposal is kept unclamped: the
+        // sim is finished unless it is resumed from its saved state.
+        h = minT(hh * f, final_step ? max_dt : cap);
         rej_once = false;
         nrej = 0;
         if (p.trace_h != nullptr && steps < p.trace_cap)
           p.trace_h[(sg.start + loc) * p.trace_cap + steps] = static_cast<double>(hh);
         ++steps;
         ++tot.acc;
-        const bool final_step = by_time && hh >= cap;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
           if (static_cast<double>(lane_pe<T, N, PS>(x, pk)) <= pe_stop) status = kSimOkPE;
@@ -574,6 +612,18 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
       }
       const uint64_t gs = sg.start + loc;
+      if (p.state_x != nullptr) {
+        T* sx = static_cast<T*>(p.state_x) + base;
+        T* sv = static_cast<T*>(p.state_v) + base;
+#pragma unroll
+        for (int q = 0; q < N3; ++q) {
+          sx[q] = x[q];
+          sv[q] = v[q];
+        }
+        p.state_h[gs] = static_cast<double>(h);
+        p.state_e0[gs] = e0;
+      }
+      tot.att += p.resume ? att - p.attempts[gs] : att;
       p.steps[gs] = steps;
       p.attempts[gs] = att;
       p.status[gs] = static_cast<uint8_t>(status);
@@ -581,7 +631,6 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
       if (p.trace_h != nullptr)
         for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
-      tot.att += att;
       active = false;
     }
   }

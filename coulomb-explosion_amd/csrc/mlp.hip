@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "coulomb_explosion.h"
+#include "mlp.hpp"
 
 namespace ce {
 int set_error(int code, const std::string& msg);
@@ -301,12 +302,11 @@ int export_all(ce_mlp* m) {
   return CE_OK;
 }
 
-// Forward pass of the batch staged in m->xin (fp32 [B][w0]); the output lands in
-// m->out (fp32 [B][wL]).  Keeps Z and the bf16 activations for backward.
-int forward(ce_mlp* m, int64_t B, bool keep) {
+// Forward pass of x (fp32 [B][w0], B <= cap) into y (fp32 [B][wL]) on stream s;
+// keep = true also stores Z for backward (the bf16 activations always stay).
+int forward(ce_mlp* m, const float* x, int64_t B, bool keep, float* y, hipStream_t s) {
   const int w0 = m->widths[0];
-  hipLaunchKernelGGL(to_bf16, dim3(blocks_for(B * w0)), dim3(256), 0, m->stream, m->xin,
-                     m->act[0], B * w0);
+  hipLaunchKernelGGL(to_bf16, dim3(blocks_for(B * w0)), dim3(256), 0, s, x, m->act[0], B * w0);
   for (int l = 0; l < m->L; ++l) {
     const int in = m->widths[l], out = m->widths[l + 1];
     GemmArgs g{};
@@ -327,9 +327,9 @@ int forward(ce_mlp* m, int64_t B, bool keep) {
       g.C32 = keep ? m->Z[l] : nullptr;
     } else {
       g.epilogue = kEpiBiasF32;
-      g.C32 = m->out;
+      g.C32 = y;
     }
-    gemm(m->stream, g);
+    gemm(s, g);
   }
   MLP_HIP(hipGetLastError());
   return CE_OK;
@@ -500,7 +500,7 @@ int ce_mlp_forward(ce_mlp* m, const float* x_host, size_t batch, float* y_host) 
     if (rc != CE_OK) return rc;
     MLP_HIP(hipMemcpyAsync(m->xin, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
                            hipMemcpyHostToDevice, m->stream));
-    rc = forward(m, B, false);
+    rc = forward(m, m->xin, B, false, m->out, m->stream);
     if (rc != CE_OK) return rc;
     MLP_HIP(hipMemcpyAsync(y_host + b0 * m->widths[m->L], m->out,
                            B * m->widths[m->L] * sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -522,7 +522,7 @@ int ce_mlp_gradients(ce_mlp* m, const float* x_host, const float* t_host, size_t
   const int wo = m->widths[m->L];
   MLP_HIP(hipMemcpyAsync(m->xin, x_host, B * m->widths[0] * sizeof(float), hipMemcpyHostToDevice, m->stream));
   MLP_HIP(hipMemcpyAsync(m->tgt, t_host, B * wo * sizeof(float), hipMemcpyHostToDevice, m->stream));
-  rc = forward(m, B, true);
+  rc = forward(m, m->xin, B, true, m->out, m->stream);
   if (rc != CE_OK) return rc;
   MLP_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
   hipLaunchKernelGGL(mse_grad, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->out,
@@ -618,7 +618,7 @@ int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed,
                        m->perm + b0, B, wi, m->xin);
     hipLaunchKernelGGL(gather_rows, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->ds_t,
                        m->perm + b0, B, wo, m->tgt);
-    rc = forward(m, B, true);
+    rc = forward(m, m->xin, B, true, m->out, m->stream);
     if (rc != CE_OK) return rc;
     hipLaunchKernelGGL(mse_grad, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->out,
                        static_cast<int64_t>(wo), m->tgt, B, wo, m->dY, m->loss);
@@ -644,3 +644,25 @@ int ce_mlp_adam_step(ce_mlp* m, float lr) {
 }
 
 }  // extern "C"
+
+namespace ce {
+
+int mlp_width(const ce_mlp* m, int layer) {
+  return (m != nullptr && layer >= 0 && layer <= m->L) ? m->widths[layer] : 0;
+}
+int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
+
+int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
+  const int64_t chunk = 262144;
+  const int64_t c = B < chunk ? B : chunk;
+  int rc = ensure_batch(m, c);
+  if (rc != CE_OK) return rc;
+  const int w0 = m->widths[0], wl = m->widths[m->L];
+  for (int64_t b0 = 0; b0 < B && rc == CE_OK; b0 += chunk) {
+    const int64_t n = (B - b0) < chunk ? (B - b0) : chunk;
+    rc = forward(m, x + b0 * w0, n, false, y + b0 * wl, s);
+  }
+  return rc;
+}
+
+}  // namespace ce

@@ -1,0 +1,19 @@
+// Internal interface of the closure network (mlp.hip) for the closure pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "coulomb_explosion.h"
+
+namespace ce {
+
+// Layer widths: mlp_width(m, 0) inputs ... mlp_width(m, mlp_layers(m)) outputs.
+int mlp_width(const ce_mlp* m, int layer);
+int mlp_layers(const ce_mlp* m);
+// y [B][w_out] = net(x [B][w_in]) for device buffers, on stream s (in chunks
+// of the network's batch workspace).
+int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s);
+
+}  // namespace ce

@@ -60,7 +60,8 @@ enum ce_sim_status {
   CE_SIM_NOT_CONVERGED = 3, /* max_steps hit first (reference: converged=0) */
   CE_SIM_REJECT_LIMIT = 4,  /* > max_rejections in one step (reference throws) */
   CE_SIM_NONFINITE = 5,     /* error norm NaN through the rejection limit */
-  CE_SIM_SAMPLER_FAIL = 6   /* device sampler could not place the atoms */
+  CE_SIM_SAMPLER_FAIL = 6,  /* device sampler could not place the atoms */
+  CE_SIM_CLOSURE = 7        /* momenta predicted by the learned closure */
 };
 
 /* Molecule: replaces the reference's Molecule (molecule.hpp:30-35) and the
@@ -96,7 +97,7 @@ typedef struct {
   int32_t h0_mode;
   int32_t max_rejections;
   int32_t precision;
-  int32_t pad_;
+  int32_t resume;  /* 1: continue every sim from the saved state (ce_engine_enable_state) */
 } ce_run_config;
 
 /* Result buffers, caller-allocated host memory; a NULL field is not copied.
@@ -215,6 +216,22 @@ int ce_engine_get_launch(const ce_engine* eng, ce_launch_config* lc);
  * runs (the reference's StepObserver hook, driver.hpp:12-18); 0 disables. */
 int ce_engine_enable_trace(ce_engine* eng, uint32_t trace_cap);
 
+/* Saved integration state (no reference counterpart; the learned closure stops
+ * sims at an early t_max and continues some of them).  When enabled, every run
+ * keeps each sim's positions and velocities at its stop (in the run's
+ * precision), the next step size and E0 on the device; a later run with
+ * cfg->resume = 1 continues each sim from there (time, step counts and the
+ * energy reference carry over), e.g. from a t_cut stop to the full stop rule. */
+int ce_engine_enable_state(ce_engine* eng, int32_t enable);
+/* Host copy of the saved state: pos/vel [sim][n][3], h and e0 per sim (any may
+ * be NULL); n_sims must equal the loaded sims. */
+int ce_engine_get_state(ce_engine* eng, double* pos, double* vel, double* h, double* e0,
+                        size_t n_sims);
+/* As ce_engine_run, for the listed sims only (single-species engines); the other
+ * sims' results are left as they were. */
+int ce_engine_run_subset(ce_engine* eng, const ce_run_config* cfg, const uint64_t* sims,
+                         size_t n, ce_results* out);
+
 /* Dataset records of the reference generator (examples/generate_dataset.cpp:
  * 198-217; numpy dtype examples/to_parquet.py:34-42): per sim, packed
  * little-endian, no padding -- n*3 f32 initial positions, n*3 f32 final momenta
@@ -247,6 +264,45 @@ int ce_mlp_adam_step(ce_mlp* m, float lr);
 int ce_mlp_set_dataset(ce_mlp* m, const float* x, const float* t, size_t n);
 int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed,
                        float* mean_loss);
+
+/* ---- learned asymptotic closure (no reference counterpart) ----------------
+ * For one molecule size (single-species engines, f32, momentum-or-t_max rule):
+ * integrate to t_cut keeping the state, predict the asymptotic momenta of the
+ * sims stopped by the cut as a closed-form straight-line-impulse prior (KE
+ * rescaled to E0) plus an MLP ensemble's residual, accept the ensemble mean
+ * where the members' per-atom spread is <= gate_tol relative to |p_i|
+ * (status CE_SIM_CLOSURE), and continue every other cut sim from its saved
+ * state to the stop rule.  Network input: 8n features per sim (per atom its
+ * position relative to the centre of mass and m v, then the n charges and n
+ * masses), shifted and scaled; output: 3n residual components divided by
+ * out_scale. */
+typedef struct ce_closure ce_closure;
+typedef struct {
+  uint64_t sims;              /* loaded sims */
+  uint64_t candidates;        /* sims stopped by the cut */
+  uint64_t fallback;          /* of those, continued by integration */
+  uint64_t cut_attempts;      /* attempted steps up to the cut */
+  uint64_t fallback_attempts; /* attempted steps of the continuation */
+  double cut_ms;              /* integration kernel to t_cut */
+  double infer_ms;            /* features + ensemble + gate */
+  double fallback_ms;         /* continuation kernel */
+  double total_ms;            /* whole ce_closure_run on the device stream */
+} ce_closure_stats;
+int ce_closure_create(int32_t n_atoms, const double* masses, const double* charges,
+                      double coulomb_k, ce_closure** out);
+int ce_closure_destroy(ce_closure* c);
+int ce_closure_set_normalization(ce_closure* c, const float* in_shift, const float* in_scale,
+                                 const float* out_scale);
+/* 2..8 members for a gate (fewer: every cut sim falls back); members are not
+ * owned and must outlive their use. */
+int ce_closure_set_members(ce_closure* c, ce_mlp* const* members, int32_t n_members);
+/* Raw features [sim][8n] and the prior [sim][n][3] of the engine's saved f32
+ * state, for every loaded sim (training data). */
+int ce_closure_features(ce_closure* c, ce_engine* eng, float* features, double* prior,
+                        size_t n_sims);
+int ce_closure_run(ce_closure* c, ce_engine* eng, const ce_run_config* cfg, double t_cut,
+                   double gate_tol, ce_results* out);
+int ce_closure_get_stats(const ce_closure* c, ce_closure_stats* out);
 
 /* Host sampler with the reference's exact stream: UniformSphereSampler(Options,
  * seed) (sampler.hpp:46-72, sampler.cpp:7-57) -- std::mt19937_64 feeding

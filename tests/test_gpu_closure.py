@@ -1,0 +1,168 @@
+"""Saved state / resume, sim subsets and the learned closure, through the C-ABI."""
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+from conftest import CH4_CHARGES, CH4_MASSES, per_atom_rel
+from oracle import closure_ref
+
+pytestmark = pytest.mark.gpu
+
+RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+T_CUT = 200.0
+
+
+def ch4(ce):
+    return ce.Molecule([ce.Atom("X", m, q) for m, q in zip(CH4_MASSES, CH4_CHARGES)])
+
+
+def engine(ce, n, seed, state=True):
+    eng = ce.BatchedRK45(ch4(ce), 1.0, n, 1e-6, 1e-9)
+    eng.sample(seed, n, 2.5, 1.0)
+    if state:
+        eng.enable_state()
+    return eng
+
+
+def pe(x, q):
+    n = len(q)
+    return sum(q[i] * q[j] / np.linalg.norm(x[i] - x[j]) for i in range(n) for j in range(i + 1, n))
+
+
+def test_cut_then_resume_matches_uninterrupted(gpu, oracle):
+    ce = gpu
+    from oracle import ffi
+    eng = engine(ce, 2000, 31)
+    cfg = ce.RunConfig(**RULE)
+    full = eng.run(cfg)
+    cut = eng.run(replace(cfg, t_max=T_CUT, redistribute_energy=False))  # momenta = m v
+    assert np.all(cut.status == ce.SIM_OK_TMAX) and np.all(cut.t_final == T_CUT)
+    # the cut state: momenta (no redistribution) against the oracle's lane run to t_cut
+    geo = eng.geometries()
+    ref = oracle.lane_run_q(np.array(CH4_MASSES), np.tile(CH4_CHARGES, (2000, 1)), geo,
+                            ffi.make_config(rtol=1e-6, atol=1e-9, stop_rule=1, t_max=T_CUT,
+                                            momentum_tol=1e-7, h0_mode=1, h0=1e-3,
+                                            redistribute=False))
+    assert per_atom_rel(cut.momenta, ref.momenta).max() < 1e-5
+    st = eng.state()
+    np.testing.assert_allclose(st["vel"] * np.array(CH4_MASSES)[None, :, None],
+                               cut.momenta.reshape(-1, 5, 3), rtol=1e-6, atol=0)
+    for s in range(0, 2000, 97):  # energy of the saved state = E0 (to the tolerance)
+        ke = 0.5 * np.sum(np.array(CH4_MASSES)[:, None] * st["vel"][s] ** 2)
+        assert (ke + pe(st["pos"][s], CH4_CHARGES)) == pytest.approx(st["e0"][s], rel=1e-4)
+        assert st["e0"][s] == pytest.approx(pe(geo[s], CH4_CHARGES), rel=1e-6)
+    assert np.all(st["h"] > 0)
+    res = eng.run(replace(cfg, resume=True))
+    assert np.all(res.t_final == 1e4)
+    assert per_atom_rel(res.momenta, full.momenta.reshape(-1, 5, 3)).max() < 1e-5
+    assert np.all(res.steps >= cut.steps)
+    assert np.abs(res.steps.astype(int) - full.steps.astype(int)).max() <= 3
+
+
+def test_resume_needs_state_of_same_precision(gpu):
+    ce = gpu
+    eng = engine(ce, 64, 1, state=False)
+    with pytest.raises(RuntimeError):
+        eng.run(ce.RunConfig(resume=True, **RULE))
+    eng.enable_state()
+    eng.run(ce.RunConfig(**dict(RULE, t_max=T_CUT)))
+    with pytest.raises(RuntimeError):
+        eng.run(ce.RunConfig(resume=True, precision=1, **RULE))
+
+
+def test_run_subset_touches_only_listed_sims(gpu):
+    ce = gpu
+    eng = engine(ce, 1000, 5)
+    cfg = ce.RunConfig(**RULE)
+    eng.run(replace(cfg, t_max=T_CUT))
+    cut = eng.download()
+    everything = eng.run(replace(cfg, resume=True))
+    eng.run(replace(cfg, t_max=T_CUT))  # same cut state again (deterministic)
+    sub = np.array([3, 10, 11, 500, 999, 0])
+    res = eng.run_subset(sub, replace(cfg, resume=True))
+    assert eng.stats()["sims"] == len(sub)
+    m = res.momenta.reshape(-1, 5, 3)
+    np.testing.assert_array_equal(m[sub], everything.momenta.reshape(-1, 5, 3)[sub])
+    rest = np.setdiff1d(np.arange(1000), sub)
+    np.testing.assert_array_equal(m[rest], cut.momenta.reshape(-1, 5, 3)[rest])
+    np.testing.assert_array_equal(res.status[rest], cut.status[rest])
+
+
+def test_features_and_prior_vs_oracle(gpu):
+    ce = gpu
+    eng = engine(ce, 500, 8)
+    eng.run(ce.RunConfig(**dict(RULE, t_max=T_CUT)))
+    st = eng.state()
+    clo = ce.Closure(ch4(ce))
+    f, pr = clo.features(eng)
+    m, q = np.array(CH4_MASSES), np.array(CH4_CHARGES)
+    for s in range(0, 500, 23):
+        x, v = st["pos"][s], st["vel"][s]
+        np.testing.assert_allclose(pr[s], closure_ref.prior(x, v, m, q, st["e0"][s]), rtol=1e-12,
+                                   atol=1e-12 * np.abs(pr[s]).max())
+        np.testing.assert_allclose(f[s], closure_ref.features(x, v, m, q), rtol=2e-7, atol=1e-6)
+
+
+@pytest.fixture(scope="module")
+def trained(gpu):
+    """A small ensemble trained on engine data (the real one is in bench.py)."""
+    ce = gpu
+    import coulomb_explosion_amd.closure_train as ct
+    cfg = ce.RunConfig(**RULE)
+    data = ct.make_data(ce, ch4(ce), 100000, 4242, cfg, T_CUT, 2.5, 1.0, 1e-6, 1e-9)
+    ens = ct.fit(ce, data, members=3, hidden=(128, 128), epochs=8, batch=1024, lr=2e-3)
+    return ct, cfg, ens
+
+
+def test_network_improves_on_prior(gpu, trained):
+    """What the trained network gains over the method without it, on fresh sims."""
+    ce = gpu
+    ct, cfg, ens = trained
+    data = ct.make_data(ce, ch4(ce), 4000, 777, cfg, T_CUT, 2.5, 1.0, 1e-6, 1e-9)
+    mean, rel = ct.predict(ens, data.features, data.prior)
+    e_prior = per_atom_rel(data.prior, data.final)
+    e_net = per_atom_rel(mean, data.final)
+    assert np.median(e_net) < 0.5 * np.median(e_prior)
+    assert rel.shape == (len(data.final),)
+
+
+def test_closure_gate_extremes(gpu, trained):
+    ce = gpu
+    ct, cfg, ens = trained
+    clo = ct.attach(ce, ch4(ce), ens)
+    eng = engine(ce, 3000, 99)
+    ref = eng.run(replace(cfg, t_max=T_CUT))
+    feat, prior = clo.features(eng)
+    resumed = eng.run(replace(cfg, resume=True))
+    # gate 0: every cut sim is continued -> the same as cut + resume
+    res0 = clo.run(eng, cfg, T_CUT, 0.0)
+    s0 = clo.stats()
+    assert s0["candidates"] == 3000 and s0["fallback"] == 3000
+    np.testing.assert_array_equal(res0.momenta, resumed.momenta)
+    np.testing.assert_array_equal(res0.status, resumed.status)
+    # gate wide open: every cut sim takes the ensemble mean
+    res1 = clo.run(eng, cfg, T_CUT, 1e30)
+    s1 = clo.stats()
+    assert s1["fallback"] == 0 and np.all(res1.status == ce.SIM_CLOSURE)
+    mean, _ = ct.predict(ens, feat, prior)
+    np.testing.assert_allclose(res1.momenta.reshape(-1, 5, 3), mean, rtol=1e-6,
+                               atol=1e-6 * np.abs(mean).max())
+    assert np.all(ref.steps == res1.steps)
+
+
+def test_closure_gate_follows_spread(gpu, trained):
+    ce = gpu
+    ct, cfg, ens = trained
+    clo = ct.attach(ce, ch4(ce), ens)
+    eng = engine(ce, 3000, 123)
+    eng.run(replace(cfg, t_max=T_CUT))
+    feat, prior = clo.features(eng)
+    _, rel = ct.predict(ens, feat, prior)
+    tol = float(np.quantile(rel, 0.5))
+    res = clo.run(eng, cfg, T_CUT, tol)
+    acc = res.status == ce.SIM_CLOSURE
+    far = np.abs(rel - tol) > 1e-3 * tol  # away from the threshold (rounding)
+    np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
+    assert clo.stats()["fallback"] == np.count_nonzero(~acc)
+    assert np.all(res.status[~acc] == ce.SIM_OK_TMAX) and np.all(res.t_final[~acc] == 1e4)
