@@ -38,90 +38,183 @@ __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
   u += 0x7FFFu + ((u >> 16) & 1u);
   return static_cast<uint16_t>(u >> 16);
 }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// sigmoid with the hardware exp2 and reciprocal (about 1 ulp each; the result is
+// rounded to bf16 or feeds a bf16 product)
+__device__ __forceinline__ float sigmoid(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+__device__ __forceinline__ float silu(float z) { return z * sigmoid(z); }
 __device__ __forceinline__ float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
-constexpr int kTM = 64, kTN = 64, kTK = 32;
-enum Epilogue : int { kEpiStoreF32 = 0, kEpiBiasSiluBf16 = 1, kEpiBiasF32 = 2, kEpiBf16 = 3 };
+constexpr int kTM = 128, kTN = 128, kTK = 32, kLdsK = kTK + 8;  // LDS row: 80 bytes
+enum Epilogue : int { kEpiStoreF32 = 0, kEpiBiasSiluBf16 = 1, kEpiBiasF32 = 2, kEpiAtomicF32 = 3 };
 
 struct GemmArgs {
   const uint16_t* A;  // bf16 bits
   const uint16_t* B;
-  int64_t sAm, sAk, sBk, sBn;  // element strides
-  int M, N, K;                 // K may exceed the stored extent only with zero padding
+  int64_t sAm, sAk, sBk, sBn;  // element strides; per operand one of them is 1
+  int M, N, K;
+  int k_split;                 // K range of one blockIdx.z (multiple of kTK)
   const float* bias;           // [N] or nullptr
-  float* C32;                  // fp32 out [M][ldc] (pre-activation / plain)
+  float* C32;                  // fp32 out [M][ldc] (pre-activation / plain / atomic sum)
   uint16_t* C16;               // bf16 out [M][ldc]
   int64_t ldc;
   int epilogue;
-  int accumulate;  // C32 += result
 };
 
-// One workgroup = 4 waves = a 64 x 64 output tile; wave w owns the 32 x 32 block
-// (w / 2, w % 2).  Per 32-deep K step: stage A (64 x 32) as As[m][k] and B
-// (32 x 64) as Bs[n][k] in LDS, then two v_mfma_f32_32x32x16_bf16 per wave.
+// Eight bf16 of one operand tile chunk, held in registers between the global
+// load and the LDS store.
+struct Chunk8 {
+  uint4 v;
+};
+
+// Operand tile (128 rows of m or n, 32 of k) from global memory into registers:
+// thread t handles chunks t and t + 256 of 8 elements.  KC: k is the contiguous
+// dimension (chunk = 8 consecutive k of one row); otherwise the row dimension is
+// contiguous (chunk = 8 consecutive rows at one k).  Out-of-range elements are 0.
+template <bool KC>
+__device__ __forceinline__ void load_tile(const uint16_t* X, int64_t s_mn, int64_t s_k, int MN,
+                                          int K, int mn0, int k0, bool vec, Chunk8 (&c)[2]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int id = tid + 256 * i;
+    int mn, k;
+    if constexpr (KC) {
+      mn = mn0 + (id >> 2);
+      k = k0 + (id & 3) * 8;
+    } else {
+      k = k0 + (id >> 4);
+      mn = mn0 + (id & 15) * 8;
+    }
+    const bool full = KC ? (mn < MN && k + 8 <= K) : (k < K && mn + 8 <= MN);
+    if (vec && full) {
+      c[i].v = *reinterpret_cast<const uint4*>(X + mn * s_mn + k * s_k);
+    } else {
+      uint16_t e[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int mj = KC ? mn : mn + j, kj = KC ? k + j : k;
+        e[j] = (mj < MN && kj < K) ? X[mj * s_mn + kj * s_k] : uint16_t(0);
+      }
+      c[i].v.x = e[0] | (uint32_t(e[1]) << 16);
+      c[i].v.y = e[2] | (uint32_t(e[3]) << 16);
+      c[i].v.z = e[4] | (uint32_t(e[5]) << 16);
+      c[i].v.w = e[6] | (uint32_t(e[7]) << 16);
+    }
+  }
+}
+
+// Registers -> LDS as T[row][k] (row = m or n, k contiguous, padded rows).
+template <bool KC>
+__device__ __forceinline__ void store_tile(uint16_t (*T)[kLdsK], const Chunk8 (&c)[2]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int id = tid + 256 * i;
+    if constexpr (KC) {
+      *reinterpret_cast<uint4*>(&T[id >> 2][(id & 3) * 8]) = c[i].v;
+    } else {
+      const int k = id >> 4, r = (id & 15) * 8;
+      const uint32_t w[4] = {c[i].v.x, c[i].v.y, c[i].v.z, c[i].v.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        T[r + 2 * j][k] = static_cast<uint16_t>(w[j] & 0xFFFFu);
+        T[r + 2 * j + 1][k] = static_cast<uint16_t>(w[j] >> 16);
+      }
+    }
+  }
+}
+
+// C (M x N) = A (M x K) B (K x N) in bf16 with FP32 accumulation.  One
+// workgroup = 4 waves = a 128 x 128 tile of C; wave w owns the 64 x 64 quarter
+// (w / 2, w % 2) as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16.  Per 32-deep K
+// step the A and B tiles are staged in LDS as [m][k] and [n][k]; the next
+// step's global loads are issued before the current step's MFMAs.
+// blockIdx.z splits K (kEpiAtomicF32 sums the parts into a zeroed C32).
 // MFMA operand layout (32x32x16): lane l holds A[l % 32][8 (l / 32) + 0..7] and
 // B[8 (l / 32) + 0..7][l % 32]; the result register r of lane l is
 // C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+template <bool AK, bool BK>
 __global__ void __launch_bounds__(256) gemm_bf16(const GemmArgs g) {
-  __shared__ __align__(16) uint16_t As[kTM][kTK + 8];
-  __shared__ __align__(16) uint16_t Bs[kTN][kTK + 8];
+  __shared__ __align__(16) uint16_t As[kTM][kLdsK];
+  __shared__ __align__(16) uint16_t Bs[kTN][kLdsK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * kTM, n0 = blockIdx.x * kTN;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f32x16 acc;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  const int kb = blockIdx.z * g.k_split;
+  const int ke = min(g.K, kb + g.k_split);
+  // 16-byte loads need the contiguous dimension's partner stride and the base aligned
+  const bool va = (reinterpret_cast<uintptr_t>(g.A) & 15) == 0 && ((AK ? g.sAm : g.sAk) % 8) == 0;
+  const bool vb = (reinterpret_cast<uintptr_t>(g.B) & 15) == 0 && ((BK ? g.sBn : g.sBk) % 8) == 0;
+  f32x16 acc[2][2];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-  for (int k0 = 0; k0 < g.K; k0 += kTK) {
-    // 64 x 32 elements of A and 32 x 64 of B, 8 per thread each.
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int idx = tid * 8 + e;  // 0..2047
-      const int r = idx / kTK, c = idx % kTK;
-      const int m = m0 + r, k = k0 + c;
-      As[r][c] = (m < g.M && k < g.K) ? g.A[m * g.sAm + k * g.sAk] : uint16_t(0);
-      const int n = n0 + r;
-      Bs[r][c] = (n < g.N && k < g.K) ? g.B[k * g.sBk + n * g.sBn] : uint16_t(0);
-    }
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  Chunk8 ra[2], rb[2];
+  if (kb < ke) {
+    load_tile<AK>(g.A, g.sAm, g.sAk, g.M, ke, m0, kb, va, ra);
+    load_tile<BK>(g.B, g.sBn, g.sBk, g.N, ke, n0, kb, vb, rb);
+  }
+  for (int k0 = kb; k0 < ke; k0 += kTK) {
+    store_tile<AK>(As, ra);
+    store_tile<BK>(Bs, rb);
     __syncthreads();
+    if (k0 + kTK < ke) {
+      load_tile<AK>(g.A, g.sAm, g.sAk, g.M, ke, m0, k0 + kTK, va, ra);
+      load_tile<BK>(g.B, g.sBn, g.sBk, g.N, ke, n0, k0 + kTK, vb, rb);
+    }
 #pragma unroll
     for (int kk = 0; kk < kTK; kk += 16) {
       const int row = lane & 31, kq = kk + 8 * (lane >> 5);
-      bf16x8 a, b;
-      const uint16_t* pa = &As[wm + row][kq];
-      const uint16_t* pb = &Bs[wn + row][kq];
+      bf16x8 a[2], b[2];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        a[j] = __builtin_bit_cast(__bf16, pa[j]);
-        b[j] = __builtin_bit_cast(__bf16, pb[j]);
+      for (int i = 0; i < 2; ++i) {
+        a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 32 * i + row][kq]);
+        b[i] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 32 * i + row][kq]);
       }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
-  const int col = n0 + wn + (lane & 31);
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = m0 + wm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (row >= g.M || col >= g.N) continue;
-    float v = acc[r];
-    if (g.bias) v += g.bias[col];
-    const int64_t o = row * g.ldc + col;
-    switch (g.epilogue) {
-      case kEpiStoreF32:
-      case kEpiBiasF32:
-        g.C32[o] = g.accumulate ? g.C32[o] + v : v;
-        break;
-      case kEpiBiasSiluBf16:
-        if (g.C32) g.C32[o] = v;
-        g.C16[o] = f32_to_bf16_bits(silu(v));
-        break;
-      case kEpiBf16:
-        g.C16[o] = f32_to_bf16_bits(v);
-        break;
+  for (int i = 0; i < 2; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int col = n0 + wn + 32 * j + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm + 32 * i + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row >= g.M || col >= g.N) continue;
+        float v = acc[i][j][r];
+        const int64_t o = row * g.ldc + col;
+        switch (g.epilogue) {
+          case kEpiStoreF32:
+            g.C32[o] = v;
+            break;
+          case kEpiBiasF32:
+            g.C32[o] = v + g.bias[col];
+            break;
+          case kEpiBiasSiluBf16:
+            v += g.bias[col];
+            if (g.C32) g.C32[o] = v;
+            g.C16[o] = f32_to_bf16_bits(silu(v));
+            break;
+          case kEpiAtomicF32:
+            atomicAdd(g.C32 + o, v);
+            break;
+        }
+      }
     }
   }
 }
@@ -213,7 +306,6 @@ struct ce_mlp {
   float* out = nullptr;        // [cap][w[L]]
   float *dY = nullptr, *dZ32 = nullptr;
   uint16_t* dZ16 = nullptr;
-  float* dX = nullptr;
   float* loss = nullptr;
   float* xin = nullptr;  // fp32 staging of inputs [cap][w[0]]
   float* tgt = nullptr;  // fp32 targets [cap][w[L]]
@@ -249,7 +341,7 @@ void free_mlp(ce_mlp* m) {
   auto f = [](void* p) {
     if (p) (void)hipFree(p);
   };
-  f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16), f(m->dX);
+  f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16);
   f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
   for (auto* p : m->Wb) f(p);
   for (auto* p : m->act) f(p);
@@ -265,7 +357,7 @@ int ensure_batch(ce_mlp* m, int64_t B) {
   };
   for (auto* p : m->act) f(p);
   for (auto* p : m->Z) f(p);
-  f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16), f(m->dX), f(m->xin), f(m->tgt);
+  f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16), f(m->xin), f(m->tgt);
   m->act.assign(m->L, nullptr);
   m->Z.assign(m->L, nullptr);
   int maxw = 0;
@@ -279,16 +371,36 @@ int ensure_batch(ce_mlp* m, int64_t B) {
   if (rc == CE_OK) rc = dalloc(&m->dY, B * maxw);
   if (rc == CE_OK) rc = dalloc(&m->dZ32, B * maxw);
   if (rc == CE_OK) rc = dalloc(&m->dZ16, B * maxw);
-  if (rc == CE_OK) rc = dalloc(&m->dX, B * maxw);
   if (rc == CE_OK) rc = dalloc(&m->xin, B * m->widths[0]);
   if (rc == CE_OK) rc = dalloc(&m->tgt, B * m->widths[m->L]);
   if (rc == CE_OK) m->cap = B;
   return rc;
 }
 
-void gemm(hipStream_t s, const GemmArgs& g) {
-  const dim3 grid(blocks_for(g.N, kTN), blocks_for(g.M, kTM));
-  hipLaunchKernelGGL(gemm_bf16, grid, dim3(256), 0, s, g);
+// Launches C = A B.  Each operand has unit stride along k (row-major A / a
+// [n][k] B) or along its m / n dimension.  With `split_k`, kEpiAtomicF32 sums
+// K slices into C32, which the caller has zeroed.
+int gemm(hipStream_t s, GemmArgs g, bool split_k = false) {
+  const bool ak = g.sAk == 1, bk = g.sBk == 1;
+  if ((!ak && g.sAm != 1) || (!bk && g.sBn != 1))
+    return fail(CE_ERR_INVALID_ARGUMENT, "gemm operands need a unit stride");
+  const unsigned gx = blocks_for(g.N, kTN), gy = blocks_for(g.M, kTM);
+  unsigned gz = 1;
+  g.k_split = ((g.K + kTK - 1) / kTK) * kTK;
+  if (split_k) {
+    // enough workgroups for the chip: ~512 over the tiles, >= 4 K steps each
+    const int steps = (g.K + kTK - 1) / kTK;
+    const int want = static_cast<int>(std::max<unsigned>(1u, 512u / (gx * gy)));
+    const int per = std::max(4, (steps + want - 1) / want);
+    g.k_split = per * kTK;
+    gz = static_cast<unsigned>((steps + per - 1) / per);
+  }
+  const dim3 grid(gx, gy, gz);
+  if (ak && bk) hipLaunchKernelGGL((gemm_bf16<true, true>), grid, dim3(256), 0, s, g);
+  else if (ak) hipLaunchKernelGGL((gemm_bf16<true, false>), grid, dim3(256), 0, s, g);
+  else if (bk) hipLaunchKernelGGL((gemm_bf16<false, true>), grid, dim3(256), 0, s, g);
+  else hipLaunchKernelGGL((gemm_bf16<false, false>), grid, dim3(256), 0, s, g);
+  return CE_OK;
 }
 
 int export_all(ce_mlp* m) {
@@ -329,7 +441,8 @@ int forward(ce_mlp* m, const float* x, int64_t B, bool keep, float* y, hipStream
       g.epilogue = kEpiBiasF32;
       g.C32 = y;
     }
-    gemm(s, g);
+    const int rc = gemm(s, g);
+    if (rc != CE_OK) return rc;
   }
   MLP_HIP(hipGetLastError());
   return CE_OK;
@@ -342,14 +455,15 @@ int backward(ce_mlp* m, int64_t B) {
   for (int l = m->L - 1; l >= 0; --l) {
     const int in = m->widths[l], out = m->widths[l + 1];
     const int64_t n = B * out;
+    const float* dz32 = m->dZ32;
     if (l + 1 < m->L) {
       hipLaunchKernelGGL(silu_backward, dim3(blocks_for(n)), dim3(256), 0, m->stream, dY, m->Z[l],
                          n, m->dZ16, m->dZ32);
-    } else {
+    } else {  // linear output layer: dZ = dY
       hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, m->stream, dY, m->dZ16, n);
-      MLP_HIP(hipMemcpyAsync(m->dZ32, dY, n * sizeof(float), hipMemcpyDeviceToDevice, m->stream));
+      dz32 = dY;
     }
-    hipLaunchKernelGGL(column_sums, dim3(out), dim3(256), 0, m->stream, m->dZ32, B, out,
+    hipLaunchKernelGGL(column_sums, dim3(out), dim3(256), 0, m->stream, dz32, B, out,
                        static_cast<int64_t>(out), m->G + m->b_off[l]);
     GemmArgs gw{};  // dW[o][i] = sum_b dZ[b][o] X[b][i]
     gw.A = m->dZ16;
@@ -363,8 +477,10 @@ int backward(ce_mlp* m, int64_t B) {
     gw.K = static_cast<int>(B);
     gw.C32 = m->G + m->w_off[l];
     gw.ldc = in;
-    gw.epilogue = kEpiStoreF32;
-    gemm(m->stream, gw);
+    gw.epilogue = kEpiAtomicF32;  // split over the batch, summed into zeroed G
+    MLP_HIP(hipMemsetAsync(gw.C32, 0, sizeof(float) * in * out, m->stream));
+    int rc = gemm(m->stream, gw, true);
+    if (rc != CE_OK) return rc;
     if (l > 0) {  // dX[b][i] = sum_o dZ[b][o] W[o][i]
       GemmArgs gx{};
       gx.A = m->dZ16;
@@ -376,12 +492,11 @@ int backward(ce_mlp* m, int64_t B) {
       gx.M = static_cast<int>(B);
       gx.N = in;
       gx.K = out;
-      gx.C32 = m->dX;
+      gx.C32 = m->dY;  // dY of this layer is consumed (dZ16 / dz32) by now
       gx.ldc = in;
       gx.epilogue = kEpiStoreF32;
-      gemm(m->stream, gx);
-      MLP_HIP(hipMemcpyAsync(m->dY, m->dX, B * in * sizeof(float), hipMemcpyDeviceToDevice,
-                             m->stream));
+      rc = gemm(m->stream, gx);
+      if (rc != CE_OK) return rc;
       dY = m->dY;
     }
   }
