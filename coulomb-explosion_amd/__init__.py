@@ -122,6 +122,8 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_engine_enable_state": ([P, C.c_int32], C.c_int),
         "ce_engine_get_state": ([P, P, P, P, P, S], C.c_int),
         "ce_engine_run_subset": ([P, P, P, S, P], C.c_int),
+        "ce_engine_capture": ([P, P, C.c_uint64, S, C.c_double, C.c_double, C.c_uint32], C.c_int),
+        "ce_engine_replay": ([P, C.c_uint64, P], C.c_int),
         "ce_engine_create_mixed": ([P, C.c_int32, C.c_double, P, P], C.c_int),
         "ce_engine_sample_species": ([P, C.c_uint64, C.c_uint64, C.c_uint32], C.c_int),
         "ce_engine_set_charges": ([P, P, S], C.c_int),
@@ -416,6 +418,23 @@ class BatchedRK45:
         c = cfg._c()
         _check(_lib.ce_engine_run_subset(self._h, C.byref(c), idx.ctypes.data, idx.size, None))
         return self.download()
+
+    def capture(self, cfg: RunConfig, seed: int, n_sims: int, radius: float,
+                min_separation: float, max_attempts: int = 10000) -> None:
+        """Record one batch (device sampling + integration) as a HIP graph."""
+        if n_sims > self.k:
+            raise ValueError("n_sims exceeds lanes")
+        c = cfg._c()
+        _check(_lib.ce_engine_capture(self._h, C.byref(c), seed, n_sims, radius, min_separation,
+                                      max_attempts))
+        self._graph_sims = int(n_sims)
+
+    def replay(self, first_index: int, download: bool = True) -> Result | None:
+        """Launch the captured batch for sims [first_index, first_index + n_sims)."""
+        _check(_lib.ce_engine_replay(self._h, first_index, None))
+        self._loaded = self._graph_sims
+        self._dirty = False
+        return self.download() if download else None
 
     def enable_state(self, enable: bool = True) -> None:
         """Keep every sim's integration state at its stop for a later resume."""

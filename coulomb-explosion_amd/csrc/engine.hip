@@ -60,6 +60,9 @@ void free_engine(ce_engine* e) {
                   e->records, e->state_x, e->state_v, e->state_h, e->state_e0, e->list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  if (e->graph) (void)hipGraphDestroy(e->graph);
+  if (e->first_dev) (void)hipFree(e->first_dev);
   hipEvent_t evs[] = {e->k0, e->k1, e->s0, e->s1, e->p0, e->p1};
   for (hipEvent_t v : evs)
     if (v) (void)hipEventDestroy(v);
@@ -218,7 +221,8 @@ void launch_sample(const ce::SampleArgs& a, hipStream_t s) {
 }
 
 int sample_segment(ce_engine* e, int s, uint64_t seed, uint64_t first_global, double radius,
-                   double min_sep, uint32_t max_attempts, bool draw_charges) {
+                   double min_sep, uint32_t max_attempts, bool draw_charges,
+                   const uint64_t* first_dev = nullptr) {
   uint64_t start, pos_off, q_off;
   e->offsets(s, &start, &pos_off, &q_off);
   ce::SampleArgs a{};
@@ -226,6 +230,7 @@ int sample_segment(ce_engine* e, int s, uint64_t seed, uint64_t first_global, do
   a.status = e->gstat + start;
   a.charges = draw_charges ? e->qsim + q_off : nullptr;
   a.first_index = first_global;
+  a.first_index_dev = first_dev;
   a.n_sims = e->sp[s].count;
   a.key0 = static_cast<uint32_t>(seed);
   a.key1 = static_cast<uint32_t>(seed >> 32);
@@ -631,16 +636,17 @@ namespace ce {
 
 // Runs the loaded sims (or the segment-local sims of the device list `list`,
 // single-species engines) with `cfg`; results stay on the device.
-int engine_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, size_t n_list) {
+// Launch request of a run over the loaded sims (or the device list).
+int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, size_t n_list,
+                ce::LaunchReq* req) {
   int rc = check_config(cfg);
   if (rc != CE_OK) return rc;
   if (cfg->resume && (e->state_x == nullptr || e->state_prec != cfg->precision))
     return fail(CE_ERR_RUNTIME, "resume needs a saved state from a run of the same precision");
   if (list != nullptr && e->n_species != 1)
     return fail(CE_ERR_INVALID_ARGUMENT, "sim lists need a single-species engine");
-  CE_HIP(hipSetDevice(e->device));
-  ce::LaunchReq req{};
-  ce::RunArgs& a = req.args;
+  *req = ce::LaunchReq{};
+  ce::RunArgs& a = req->args;
   a.geom = e->geom;
   a.geom_status = e->sampled ? e->gstat : nullptr;
   a.qsim = e->have_charges ? e->qsim : nullptr;
@@ -682,34 +688,39 @@ int engine_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, siz
     e->offsets(s, &start, &pos_off, &q_off);
     a.seg[s] = ce::Segment{start, list != nullptr ? n_list : e->sp[s].count, pos_off, q_off,
                            e->sp[s].n, e->have_charges ? 1 : 0};
-    req.species[s].n = e->sp[s].n;
+    req->species[s].n = e->sp[s].n;
     for (int i = 0; i < e->sp[s].n; ++i) {
-      req.species[s].masses[i] = e->sp[s].masses[i];
-      req.species[s].charges[i] = e->sp[s].charges[i];
+      req->species[s].masses[i] = e->sp[s].masses[i];
+      req->species[s].charges[i] = e->sp[s].charges[i];
     }
   }
-  req.coulomb_k = e->coulomb_k;
-  req.per_sim_charges = e->have_charges;
-  req.block_threads = e->launch.block_threads;
-  req.blocks_per_cu = e->launch.blocks_per_cu;
-  req.n_cus = e->n_cus;
-  req.stream = e->stream;
-  // Clear the queue heads and totals with a kernel (no memset node, so the
-  // sequence stays valid inside a captured graph).
+  req->coulomb_k = e->coulomb_k;
+  req->per_sim_charges = e->have_charges;
+  req->block_threads = e->launch.block_threads;
+  req->blocks_per_cu = e->launch.blocks_per_cu;
+  req->n_cus = e->n_cus;
+  req->stream = e->stream;
+  return CE_OK;
+}
+
+// Counter reset and integrator launch(es) on the engine stream: kernels only
+// (no memset / copy nodes), so the sequence can be captured into a graph.
+int launch_run(ce_engine* e, const ce::LaunchReq& req, int precision, size_t n_run,
+               ce::LaunchInfo* info) {
   hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, e->stream, e->ctr, ce::kCtrWords);
   CE_HIP(hipGetLastError());
-  CE_HIP(hipEventRecord(e->k0, e->stream));
-  ce::LaunchInfo info;
   bool bad_n = false;
-  const size_t n_run = list != nullptr ? n_list : e->loaded();
   if (n_run > 0) {
-    const hipError_t he = (cfg->precision == CE_PRECISION_F64)
-                              ? ce::launch_dp5_f64(req, &info, &bad_n)
-                              : ce::launch_dp5_f32(req, &info, &bad_n);
+    const hipError_t he = (precision == CE_PRECISION_F64) ? ce::launch_dp5_f64(req, info, &bad_n)
+                                                          : ce::launch_dp5_f32(req, info, &bad_n);
     if (bad_n) return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
     if (he != hipSuccess) return fail(CE_ERR_HIP, std::string("dp5 launch: ") + hipGetErrorString(he));
   }
-  CE_HIP(hipEventRecord(e->k1, e->stream));
+  return CE_OK;
+}
+
+// After the k0 / k1 events bracketing a run: counters, timing and stats.
+int finish_run(ce_engine* e, size_t n_run, const ce::LaunchInfo& info, int precision) {
   unsigned long long host_ctr[ce::kCtrWords] = {};
   CE_HIP(hipMemcpyAsync(host_ctr, e->ctr, sizeof(host_ctr), hipMemcpyDeviceToHost, e->stream));
   CE_HIP(hipStreamSynchronize(e->stream));
@@ -725,13 +736,90 @@ int engine_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, siz
   e->stats.n_cus = e->n_cus;
   e->stats.launches = info.launches;
   e->have_results = true;
-  if (e->state_x != nullptr) e->state_prec = cfg->precision;
+  if (e->state_x != nullptr) e->state_prec = precision;
   return CE_OK;
+}
+
+int engine_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, size_t n_list) {
+  CE_HIP(hipSetDevice(e->device));
+  ce::LaunchReq req;
+  int rc = prepare_run(e, cfg, list, n_list, &req);
+  if (rc != CE_OK) return rc;
+  const size_t n_run = list != nullptr ? n_list : e->loaded();
+  ce::LaunchInfo info;
+  CE_HIP(hipEventRecord(e->k0, e->stream));
+  rc = launch_run(e, req, cfg->precision, n_run, &info);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipEventRecord(e->k1, e->stream));
+  return finish_run(e, n_run, info, cfg->precision);
 }
 
 }  // namespace ce
 
 extern "C" {
+
+int ce_engine_capture(ce_engine* e, const ce_run_config* cfg, uint64_t seed, size_t n_sims,
+                      double radius, double min_sep, uint32_t max_attempts) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (e->mixed || e->n_species != 1)
+    return fail(CE_ERR_INVALID_ARGUMENT, "graph capture needs a single-species engine");
+  if (!(radius > 0.0) || !(min_sep >= 0.0) || max_attempts == 0)
+    return fail(CE_ERR_INVALID_ARGUMENT, "bad sampler parameters");
+  if (n_sims > e->capacity) return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds capacity");
+  if (cfg != nullptr && cfg->resume) return fail(CE_ERR_INVALID_ARGUMENT, "a captured batch samples: no resume");
+  CE_HIP(hipSetDevice(e->device));
+  CE_HIP(hipStreamSynchronize(e->stream));
+  if (e->gexec) CE_HIP(hipGraphExecDestroy(e->gexec));
+  if (e->graph) CE_HIP(hipGraphDestroy(e->graph));
+  e->gexec = nullptr;
+  e->graph = nullptr;
+  if (e->first_dev == nullptr) {
+    const int arc = hip_alloc(&e->first_dev, 1);
+    if (arc != CE_OK) return arc;
+  }
+  e->sp[0].count = n_sims;
+  e->have_charges = false;
+  e->sampled = true;
+  ce::LaunchReq req;
+  int rc = ce::prepare_run(e, cfg, nullptr, 0, &req);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+  rc = sample_segment(e, 0, seed, 0, radius, min_sep, max_attempts, false, e->first_dev);
+  ce::LaunchInfo info;
+  if (rc == CE_OK) rc = ce::launch_run(e, req, cfg->precision, n_sims, &info);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(e->stream, &g);
+  if (rc != CE_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(CE_ERR_HIP, std::string("capture: ") + hipGetErrorString(ec));
+  e->graph = g;
+  CE_HIP(hipGraphInstantiate(&e->gexec, g, nullptr, nullptr, 0));
+  e->graph_info = info;
+  e->graph_cfg = *cfg;
+  e->graph_sims = n_sims;
+  e->have_results = false;
+  return CE_OK;
+}
+
+int ce_engine_replay(ce_engine* e, uint64_t first_index, ce_results* out) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (e->gexec == nullptr) return fail(CE_ERR_RUNTIME, "nothing captured: call ce_engine_capture");
+  CE_HIP(hipSetDevice(e->device));
+  e->sp[0].count = e->graph_sims;
+  e->have_charges = false;
+  e->sampled = true;
+  hipLaunchKernelGGL(ce::set_u64, dim3(1), dim3(64), 0, e->stream, e->first_dev, first_index);
+  CE_HIP(hipGetLastError());
+  CE_HIP(hipEventRecord(e->k0, e->stream));
+  CE_HIP(hipGraphLaunch(e->gexec, e->stream));
+  CE_HIP(hipEventRecord(e->k1, e->stream));
+  const int rc = ce::finish_run(e, e->graph_sims, e->graph_info, e->graph_cfg.precision);
+  if (rc != CE_OK) return rc;
+  e->stats.sampler_ms = 0.0;  // inside the graph, in kernel_ms
+  return copy_results(e, out);
+}
 
 int ce_engine_download(ce_engine* e, ce_results* out) {
   if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");

@@ -7,6 +7,7 @@
 #include <string>
 
 #include "coulomb_explosion.h"
+#include "dispatch.hpp"
 #include "kernels.hpp"
 
 namespace ce {
@@ -66,6 +67,13 @@ struct ce_engine {
   double* state_e0 = nullptr;
   int state_prec = -1;        // precision of the run that wrote the state, -1 none
   uint32_t* list = nullptr;   // sim list of ce_engine_run_subset [capacity]
+  // captured batch (ce_engine_capture): sampler + counter reset + integrator
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t* first_dev = nullptr;  // first sim index of the replayed batch
+  ce_run_config graph_cfg{};
+  size_t graph_sims = 0;
+  ce::LaunchInfo graph_info{};
   // launch
   ce_launch_config launch = {0, 0, 4, 64};
   int n_cus = 0;

@@ -705,6 +705,7 @@ struct SampleArgs {
   uint8_t* status;     // 0 ok / kSimSamplerFail, this segment's first sim
   float* charges;      // [sim][n] per-sim charges or nullptr
   uint64_t first_index;  // global index of the segment's first sim
+  const uint64_t* first_index_dev;  // if set, added to first_index (graph replays)
   uint64_t n_sims;
   uint32_t key0, key1;
   float R, twoR, R2, min2;
@@ -723,9 +724,11 @@ struct SampleArgs {
 template <int N>
 __global__ void __launch_bounds__(256) sample_sphere(const SampleArgs a) {
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  const uint64_t first =
+      a.first_index + (a.first_index_dev != nullptr ? *a.first_index_dev : uint64_t(0));
   for (uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; s < a.n_sims;
        s += stride) {
-    const uint64_t g = a.first_index + s;
+    const uint64_t g = first + s;
     // One candidate per iteration, so a wave iterates until its slowest lane
     // has placed every atom (not atom by atom).  The draw sequence is the same
     // as placing atom by atom: candidate j, cube rejection (at most 4096 draws
@@ -913,6 +916,10 @@ __global__ void __launch_bounds__(kRecBlock) pack_records(const double* geom, co
   for (uint64_t i = threadIdx.x; i < n16; i += kRecBlock)
     reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(stage)[i];
   for (uint64_t i = n16 * 16 + threadIdx.x; i < bytes; i += kRecBlock) dst[i] = stage[i];
+}
+
+static __global__ void set_u64(uint64_t* p, uint64_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 static __global__ void zero_u64(unsigned long long* p, int n) {

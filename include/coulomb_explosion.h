@@ -207,6 +207,16 @@ int ce_engine_get_geometries(ce_engine* eng, double* pos, size_t n_sims);
  * integrates every loaded sim to its stop and copies the requested results
  * into `out` (may be NULL: results stay on the device). */
 int ce_engine_run(ce_engine* eng, const ce_run_config* cfg, ce_results* out);
+/* One batch as a HIP graph (no reference counterpart; the reference generator
+ * samples and integrates batch by batch, examples/generate_dataset.cpp:180-217):
+ * ce_engine_capture records device sampling of n_sims sims (as ce_engine_sample),
+ * the counter reset and the integration with cfg on the engine's stream, with
+ * no memset or copy nodes; ce_engine_replay launches it for sims
+ * [first_index, first_index + n_sims) and copies results like ce_engine_run.
+ * Single-species engines; a new capture replaces the old one. */
+int ce_engine_capture(ce_engine* eng, const ce_run_config* cfg, uint64_t seed, size_t n_sims,
+                      double radius, double min_sep, uint32_t max_attempts);
+int ce_engine_replay(ce_engine* eng, uint64_t first_index, ce_results* out);
 /* Copies the last run's results (same buffers as ce_engine_run). */
 int ce_engine_download(ce_engine* eng, ce_results* out);
 int ce_engine_stats(const ce_engine* eng, ce_run_stats* out);
