@@ -9,7 +9,7 @@
 //       [--min-separation D] [--rtol T] [--atol A] [--pe-stop P]
 //       [--stop-rule pe|tmax] [--t-max T] [--momentum-tol M] [--h0 H (0 = Hairer)]
 //       [--batch B] [--sampler device|host] [--masses m1,m2,.. --charges q1,q2,..]
-//       [--progress-every B]
+//       [--progress-every B] [--no-graph]
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
@@ -45,6 +45,7 @@ struct Args {
   std::vector<std::string> symbols;
   std::string out;
   std::size_t progress_every = 1;
+  bool graph = true;  // replay full device-sampled batches from a captured HIP graph
 };
 
 std::vector<double> parse_list(const std::string& s) {
@@ -68,6 +69,10 @@ Args parse_args(int argc, char** argv) {
   Args a;
   for (int i = 1; i < argc; ++i) {
     const std::string f = argv[i];
+    if (f == "--no-graph") {
+      a.graph = false;
+      continue;
+    }
     if (i + 1 >= argc) die("unknown or incomplete argument: " + f);
     const std::string v = argv[++i];
     if (f == "--sims") a.sims = std::stoull(v);
@@ -228,18 +233,28 @@ int main(int argc, char** argv) {
   Writer writer(a.out);
   std::size_t done = 0, n_failed = 0, b = 0;
   const auto t0 = std::chrono::steady_clock::now();
+  // Full device-sampled batches replay one captured HIP graph (sampler +
+  // integrator); a final partial batch and host-sampled batches run eagerly.
+  const bool graph = !hs && a.graph && a.sims >= batch;
+  if (graph)
+    check(ce_engine_capture(eng, &cfg, a.seed, batch, a.radius, a.min_separation, 10000),
+          "ce_engine_capture");
   while (done < a.sims) {
     const std::size_t cnt = std::min(batch, a.sims - done);
-    if (hs) {
-      check(ce_host_sampler_sample(hs, n, cnt, host_pos.data()), "ce_host_sampler_sample");
-      check(ce_engine_set_geometries(eng, host_pos.data(), cnt), "ce_engine_set_geometries");
-    } else {
-      check(ce_engine_sample(eng, a.seed, done, cnt, a.radius, a.min_separation, 10000),
-            "ce_engine_sample");
-    }
     ce_results res{};
     res.status = status.data();
-    check(ce_engine_run(eng, &cfg, &res), "ce_engine_run");
+    if (graph && cnt == batch) {
+      check(ce_engine_replay(eng, done, &res), "ce_engine_replay");
+    } else {
+      if (hs) {
+        check(ce_host_sampler_sample(hs, n, cnt, host_pos.data()), "ce_host_sampler_sample");
+        check(ce_engine_set_geometries(eng, host_pos.data(), cnt), "ce_engine_set_geometries");
+      } else {
+        check(ce_engine_sample(eng, a.seed, done, cnt, a.radius, a.min_separation, 10000),
+              "ce_engine_sample");
+      }
+      check(ce_engine_run(eng, &cfg, &res), "ce_engine_run");
+    }
     for (std::size_t s = 0; s < cnt; ++s) n_failed += status[s] > CE_SIM_OK_TMAX ? 1 : 0;
     std::vector<uint8_t>& buf = bufs[b % 2];
     check(ce_engine_pack_records(eng, 0, cnt, buf.data()), "ce_engine_pack_records");

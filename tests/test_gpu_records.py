@@ -38,12 +38,16 @@ def test_records_match_results(gpu, oracle):
     assert raw[124] == rec["converged"][0]
 
 
-def test_generator_cli(gpu, tmp_path):
+@pytest.mark.parametrize("graph", [True, False])
+def test_generator_cli(gpu, tmp_path, graph):
+    """Two full batches (replayed from a captured graph unless --no-graph) and a
+    partial one give the records of one 5000-sim run."""
     exe = os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset")
     out = str(tmp_path / "ch4.bin")
     r = subprocess.run([exe, "--out", out, "--sims", "5000", "--batch", "2048", "--radius", "2.5",
                         "--min-separation", "1.0", "--rtol", "1e-6", "--atol", "1e-9",
-                        "--stop-rule", "tmax", "--h0", "1e-3", "--seed", "11"],
+                        "--stop-rule", "tmax", "--h0", "1e-3", "--seed", "11"] +
+                       ([] if graph else ["--no-graph"]),
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     meta = json.load(open(out + ".meta.json"))
