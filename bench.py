@@ -52,6 +52,10 @@ RTOL, ATOL, H0, T_MAX, MOM_TOL = 1e-6, 1e-9, 1e-3, 1e4, 1e-7
 RADIUS, MIN_SEP = 2.5, 1.0
 SEED = 49188291
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
+# HBM bytes per sim of one integrator launch, from the FETCH_SIZE / WRITE_SIZE passes
+# scaled by the plain-copy calibration (x2.0 read, x1.0 write), 2M-sim CH4 launch:
+# 122.7 B read + 224.9 B written (profiles/dp5_persistent_ch4_counters.md).
+MEASURED_HBM_BYTES_PER_SIM = {"ch4": 122.7 + 224.9}
 MIXED = [([N14, N14], 2), ([C_, O_, O_], 3), (CH4_MASSES, 5), ([C_, C_] + [H_] * 6, 8)]
 N10_MASSES = [C_, C_, O_, O_] + [H_] * 6
 
@@ -461,7 +465,8 @@ def main() -> int:
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": (MEASURED_HBM_BYTES_PER_SIM[args.workload] * args.sims
+                        if args.workload in MEASURED_HBM_BYTES_PER_SIM else None),
             "algorithmic_bytes_per_launch": bytes_launch,
             "hbm_gbs_algorithmic": bytes_launch / avg_kernel_s / 1e9,
             "flops_per_attempt": {str(n): flops_per_attempt(n) for n, _ in w.sizes},
