@@ -103,7 +103,7 @@ struct PairK {
 
 // Accelerations a(x) of one lane (x, a: [3N] atom-major), optionally the PE.
 // Pair term: r = x_i - x_j, d2 = r.r (fma), inv = 1/|r|, s = k q_i q_j inv^3,
-// f = r s accumulated +f on i, -f on j (Newton 3), then a_i *= 1/m_i.
+// a_i += r s and a_j -= r s as single-rounding FMAs (Newton 3), then a_i *= 1/m_i.
 template <class T, int N, bool PS, bool WITH_PE, bool WITH_IM = true>
 __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T& pe,
                                            const PairK<T, N, PS>& pk) {
@@ -123,13 +123,12 @@ __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T
       const T inv3 = (inv * inv) * inv;
       const T s = c * inv3;
       if constexpr (WITH_PE) u = fmaT(c, inv, u);
-      const T fx = rx * s, fy = ry * s, fz = rz * s;
-      a[3 * i] += fx;
-      a[3 * i + 1] += fy;
-      a[3 * i + 2] += fz;
-      a[3 * j] -= fx;
-      a[3 * j + 1] -= fy;
-      a[3 * j + 2] -= fz;
+      a[3 * i] = fmaT(rx, s, a[3 * i]);
+      a[3 * i + 1] = fmaT(ry, s, a[3 * i + 1]);
+      a[3 * i + 2] = fmaT(rz, s, a[3 * i + 2]);
+      a[3 * j] = fmaT(-rx, s, a[3 * j]);
+      a[3 * j + 1] = fmaT(-ry, s, a[3 * j + 1]);
+      a[3 * j + 2] = fmaT(-rz, s, a[3 * j + 2]);
     });
   });
   if constexpr (WITH_IM) {
@@ -363,6 +362,67 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
   bool exhausted = false;
 
   for (;;) {
+    // ---- finalize stopped lanes: redistribution and momenta in FP64 ----
+    // Batched like the refill: the (divergent) finalize runs when enough lanes
+    // have stopped to refill, or when no lane is still running, instead of on
+    // almost every iteration of a 64-lane wave.
+    const uint64_t stopped = __ballot(active && status != kRunning);
+    const bool fin_now =
+        stopped != 0 && (__ballot(active && status == kRunning) == 0 ||
+                         __popcll(stopped) + __popcll(__ballot(!active)) >= p.refill_threshold);
+    if (fin_now && active && status != kRunning) {
+      double ke = 0.0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+        ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
+      }
+      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0 / ke) : 1.0;
+      double pe_end = 0.0;
+      static_for<0, N>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        static_for<i + 1, N>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
+          const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
+          const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
+          pe_end += pk.template getd<i, j>() / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+        });
+      });
+      const uint64_t base = sg.pos_off + loc * N3;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const double ms = mc.md[i] * s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double pm = ms * static_cast<double>(v[3 * i + c]);
+          p.mom[base + 3 * i + c] = pm;
+          p.mom32[base + 3 * i + c] = static_cast<float>(pm);
+        }
+      }
+      const uint64_t gs = sg.start + loc;
+      if (p.state_x != nullptr) {
+        T* sx = static_cast<T*>(p.state_x) + base;
+        T* sv = static_cast<T*>(p.state_v) + base;
+#pragma unroll
+        for (int q = 0; q < N3; ++q) {
+          sx[q] = x[q];
+          sv[q] = v[q];
+        }
+        p.state_h[gs] = static_cast<double>(h);
+        p.state_e0[gs] = e0;
+      }
+      tot.att += p.resume ? att - p.attempts[gs] : att;
+      p.steps[gs] = steps;
+      p.attempts[gs] = att;
+      p.status[gs] = static_cast<uint8_t>(status);
+      p.t_final[gs] = t;
+      p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+      if (p.trace_h != nullptr)
+        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
+      active = false;
+    }
+
     // ---- refill idle lanes from the work queue (wave-uniform control) ----
     const uint64_t idle = __ballot(!active);
     const int n_idle = __popcll(idle);
@@ -580,59 +640,6 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       }
     }
 
-    // ---- finalize lanes that stopped: redistribution and momenta in FP64 ----
-    if (active && status != kRunning) {
-      double ke = 0.0;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
-        ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
-      }
-      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0 / ke) : 1.0;
-      double pe_end = 0.0;
-      static_for<0, N>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        static_for<i + 1, N>([&](auto J) {
-          constexpr int j = decltype(J)::value;
-          const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
-          const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
-          const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
-          pe_end += pk.template getd<i, j>() / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
-        });
-      });
-      const uint64_t base = sg.pos_off + loc * N3;
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const double ms = mc.md[i] * s;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double pm = ms * static_cast<double>(v[3 * i + c]);
-          p.mom[base + 3 * i + c] = pm;
-          p.mom32[base + 3 * i + c] = static_cast<float>(pm);
-        }
-      }
-      const uint64_t gs = sg.start + loc;
-      if (p.state_x != nullptr) {
-        T* sx = static_cast<T*>(p.state_x) + base;
-        T* sv = static_cast<T*>(p.state_v) + base;
-#pragma unroll
-        for (int q = 0; q < N3; ++q) {
-          sx[q] = x[q];
-          sv[q] = v[q];
-        }
-        p.state_h[gs] = static_cast<double>(h);
-        p.state_e0[gs] = e0;
-      }
-      tot.att += p.resume ? att - p.attempts[gs] : att;
-      p.steps[gs] = steps;
-      p.attempts[gs] = att;
-      p.status[gs] = static_cast<uint8_t>(status);
-      p.t_final[gs] = t;
-      p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
-      if (p.trace_h != nullptr)
-        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
-      active = false;
-    }
   }
 }
 

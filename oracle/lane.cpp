@@ -143,13 +143,12 @@ void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe, bool apply_im = tr
       const T inv3 = (inv * inv) * inv;
       const T s = m.pc[i][j] * inv3;
       if (pe) u = std::fma(m.pc[i][j], inv, u);
-      const T fx = rx * s, fy = ry * s, fz = rz * s;
-      a[3 * i] += fx;
-      a[3 * i + 1] += fy;
-      a[3 * i + 2] += fz;
-      a[3 * j] -= fx;
-      a[3 * j + 1] -= fy;
-      a[3 * j + 2] -= fz;
+      a[3 * i] = std::fma(rx, s, a[3 * i]);
+      a[3 * i + 1] = std::fma(ry, s, a[3 * i + 1]);
+      a[3 * i + 2] = std::fma(rz, s, a[3 * i + 2]);
+      a[3 * j] = std::fma(-rx, s, a[3 * j]);
+      a[3 * j + 1] = std::fma(-ry, s, a[3 * j + 1]);
+      a[3 * j + 2] = std::fma(-rz, s, a[3 * j + 2]);
     }
   if (apply_im)
     for (int i = 0; i < n; ++i) {
