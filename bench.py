@@ -22,7 +22,10 @@ Other workloads (--workload):
          epochs) trained on 2M engine trajectories (untimed), then 8M fresh sims
          per step: plain DP5(4) to t_cut = 200, prior + ensemble residual where
          the members agree to 1e-5, integration continued elsewhere; timed
-         beside the plain method on the same sims, which also judges the error.
+         beside the plain method on the same sims, which also judges the error;
+         a 50k subset is also compared with the engine's FP64 path.
+  closure10  the same for N=10 (2 C, 2 O, 6 H; charges uniform {1,2,3}; R 3.5,
+         min sep 0.9): 80 -> 512 x 4 -> 30, 4M training trajectories, 5M sims.
 
 Also reported: the error of the f32 results against the engine's FP64 path at
 rtol 1e-10 on a subset, the integrator kernel's achieved FP32 rate against the
@@ -194,31 +197,59 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
     }
 
 
+CLOSURE_SETUPS = {
+    # masses, charge levels (0 = all +1), R, min sep, hidden widths, train / eval sims
+    "closure": dict(masses=CH4_MASSES, levels=0, radius=RADIUS, min_sep=MIN_SEP,
+                    hidden=(256, 256, 256), train=2_000_000, eval=8_000_000, tune="ch4",
+                    name="CH4"),
+    "closure10": dict(masses=N10_MASSES, levels=3, radius=3.5, min_sep=0.9,
+                      hidden=(512, 512, 512, 512), train=4_000_000, eval=5_000_000, tune="n10",
+                      name="N=10 (2C, 2O, 6H), charges uniform {1,2,3}"),
+}
+
+
 def run_closure(ce, args) -> dict:
     import coulomb_explosion_amd.closure_train as ct
     log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
-    mol = ce.Molecule([ce.Atom("C", C_, 1.0)] + [ce.Atom("H", H_, 1.0)] * 4)
+    su = CLOSURE_SETUPS[args.workload]
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in su["masses"]])
+    nat = mol.size()
     cfg = run_config(ce)
-    log(f"[bench] closure: generating {args.train_sims} training trajectories")
-    data = ct.make_data(ce, mol, args.train_sims, SEED + 1, cfg, args.t_cut, RADIUS, MIN_SEP,
-                        RTOL, ATOL)
-    log(f"[bench] closure: {len(data.final)} pairs in {data.seconds:.1f} s; training "
+    train_sims = args.train_sims or su["train"]
+    n = args.eval_sims or su["eval"]
+    levels = su["levels"]
+    log(f"[bench] {args.workload}: generating {train_sims} training trajectories")
+    data = ct.make_data(ce, mol, train_sims, SEED + 1, cfg, args.t_cut, su["radius"],
+                        su["min_sep"], RTOL, ATOL, charge_levels=levels)
+    log(f"[bench] {args.workload}: {len(data.final)} pairs in {data.seconds:.1f} s; training "
         f"{args.members} members x {args.epochs} epochs")
-    ens = ct.fit(ce, data, members=args.members, hidden=(256, 256, 256), epochs=args.epochs,
+    ens = ct.fit(ce, data, members=args.members, hidden=su["hidden"], epochs=args.epochs,
                  batch=args.batch, lr=args.lr, seed=1, log=log)
     clo = ct.attach(ce, mol, ens)
-    n = args.eval_sims
-    eng = ce.BatchedRK45(mol, 1.0, n, RTOL, ATOL)
-    load_autotune(ce).apply_cached(ce, eng, cfg, key_extra="ch4")
-    eng.enable_state()
     seed = SEED + 2
 
+    def make_engine(count, rtol, atol):
+        if levels:
+            return ce.MixedBatch([ce.Species(mol, count, su["radius"], su["min_sep"], levels,
+                                             levels * nat)], 1.0, rtol, atol)
+        return ce.BatchedRK45(mol, 1.0, count, rtol, atol)
+
+    def sample(e, count):
+        if levels:
+            e.sample(seed)
+        else:
+            e.sample(seed, count, su["radius"], su["min_sep"])
+
+    eng = make_engine(n, RTOL, ATOL)
+    load_autotune(ce).apply_cached(ce, eng, cfg, key_extra=su["tune"])
+    eng.enable_state()
+
     def plain():
-        eng.sample(seed, n, RADIUS, MIN_SEP)
+        sample(eng, n)
         eng.run(cfg, download=False)
 
     def closure():
-        eng.sample(seed, n, RADIUS, MIN_SEP)
+        sample(eng, n)
         clo.run(eng, cfg, args.t_cut, args.gate_tol, download=False)
 
     for _ in range(args.warmup):
@@ -226,47 +257,63 @@ def run_closure(ce, args) -> dict:
         closure()
     times = {}
     for name, fn in (("plain", plain), ("closure", closure)):
-        log(f"[bench] closure: timing {name} x {args.steps}")
+        log(f"[bench] {args.workload}: timing {name} x {args.steps}")
         t0 = time.perf_counter()
         for _ in range(args.steps):
             fn()
         times[name] = (time.perf_counter() - t0) / args.steps
         if name == "plain":
-            ref = eng.download()
+            ref = ct._flat(eng.download())
             plain_kernel_ms = eng.stats()["kernel_ms"]
-    res = eng.download()
+    res = ct._flat(eng.download())
     st = clo.stats()
-    p_ref = ref.momenta.reshape(n, 5, 3)
-    err = (np.linalg.norm(res.momenta.reshape(n, 5, 3) - p_ref, axis=-1) /
-           np.linalg.norm(p_ref, axis=-1)).max(-1)
-    acc = res.status == ce.SIM_CLOSURE
-    ok = (err <= 1e-5) & (res.converged == 1)
+    p_ref = ref["momenta"].reshape(n, nat, 3)
+    p_clo = res["momenta"].reshape(n, nat, 3)
+
+    def rel(p, q):
+        return (np.linalg.norm(p - q, axis=-1) / np.linalg.norm(q, axis=-1)).max(-1)
+
+    err = rel(p_clo, p_ref)
+    acc = res["status"] == ce.SIM_CLOSURE
+    ok = (err <= 1e-5) & ((res["status"] <= ce.SIM_OK_TMAX) | acc)
+    fp64 = None
+    if args.fp64_subset:
+        # the first fp64_subset sims again on the engine's FP64 path (rtol 1e-10)
+        e64 = make_engine(args.fp64_subset, 1e-10, 1e-14)
+        sample(e64, args.fp64_subset)
+        r64 = ct._flat(e64.run(run_config(ce, precision=1)))
+        k = args.fp64_subset
+        p64 = r64["momenta"].reshape(k, nat, 3)
+        e_clo, e_pl = rel(p_clo[:k], p64), rel(p_ref[:k], p64)
+        fp64 = {"subset": k, "closure_within_1e-5": float(np.mean(e_clo <= 1e-5)),
+                "closure_max": float(e_clo.max()), "plain_within_1e-5": float(np.mean(e_pl <= 1e-5)),
+                "plain_max": float(e_pl.max())}
     # held-out quality of the network itself, gate aside
-    test = ct.make_data(ce, mol, 20000, SEED + 3, cfg, args.t_cut, RADIUS, MIN_SEP, RTOL, ATOL)
+    test = ct.make_data(ce, mol, 20000, SEED + 3, cfg, args.t_cut, su["radius"], su["min_sep"],
+                        RTOL, ATOL, charge_levels=levels)
     mean, spread = ct.predict(ens, test.features, test.prior)
-
-    def rel(p):
-        return (np.linalg.norm(p - test.final, axis=-1) / np.linalg.norm(test.final, axis=-1)).max(-1)
-
-    cur = test.features.reshape(len(test.final), -1)[:, :30].reshape(-1, 5, 6)[:, :, 3:]
+    cur = test.features[:, :6 * nat].reshape(-1, nat, 6)[:, :, 3:]
     quality = {"held_out_sims": len(test.final),
-               "median_err_current_momenta": float(np.median(rel(cur))),
-               "median_err_prior": float(np.median(rel(test.prior))),
-               "median_err_prior_plus_network": float(np.median(rel(mean))),
-               "p90_err_prior_plus_network": float(np.quantile(rel(mean), 0.9)),
+               "median_err_current_momenta": float(np.median(rel(cur, test.final))),
+               "median_err_prior": float(np.median(rel(test.prior, test.final))),
+               "median_err_prior_plus_network": float(np.median(rel(mean, test.final))),
+               "p90_err_prior_plus_network": float(np.quantile(rel(mean, test.final), 0.9)),
                "median_ensemble_spread": float(np.median(spread))}
-    mlp_flops = 2 * sum(a * b for a, b in zip([40, 256, 256, 256], [256, 256, 256, 15]))
+    widths = [8 * nat, *su["hidden"], 3 * nat]
+    mlp_flops = 2 * sum(a * b for a, b in zip(widths[:-1], widths[1:]))
     infer_flops = st["candidates"] * mlp_flops * args.members
+    infer_tf = infer_flops / (st["infer_ms"] / 1e3) / 1e12 if st["infer_ms"] else 0.0
     return {
         "n": n,
         "closure_s": times["closure"], "plain_s": times["plain"],
         "within": int(ok.sum()),
         "line_config": {
-            "workload": (f"CH4 learned closure, {n} fresh sims per step (R 2.5, min sep 1.0, "
-                         f"rtol 1e-6, atol 1e-9, h0 1e-3, stop |dp|/|p| < 1e-7 or t = 1e4): "
-                         f"plain DP5(4) to t_cut {args.t_cut:g}, straight-line impulse prior + "
-                         f"{args.members}-member MLP ensemble residual, gate {args.gate_tol:g} on "
-                         f"the members' per-atom spread, integration continued elsewhere"),
+            "workload": (f"{su['name']} learned closure, {n} fresh sims per step (R "
+                         f"{su['radius']}, min sep {su['min_sep']}, rtol 1e-6, atol 1e-9, h0 1e-3, "
+                         f"stop |dp|/|p| < 1e-7 or t = 1e4): plain DP5(4) to t_cut "
+                         f"{args.t_cut:g}, straight-line impulse prior + {args.members}-member MLP "
+                         f"ensemble residual, gate {args.gate_tol:g} on the members' per-atom "
+                         f"spread, integration continued elsewhere"),
             "sims_per_step": n,
             "method": "closure",
             "plain_sims_per_s": n / times["plain"],
@@ -280,6 +327,7 @@ def run_closure(ce, args) -> dict:
                 "accepted_max": float(err[acc].max()) if acc.any() else None,
                 "accepted_median": float(np.median(err[acc])) if acc.any() else None,
                 "all_max": float(err.max())},
+            "err_vs_fp64": fp64,
             "phase_ms": {"cut": st["cut_ms"], "features_ensemble_gate": st["infer_ms"],
                          "fallback": st["fallback_ms"], "total_device": st["total_ms"],
                          "plain_kernel": plain_kernel_ms},
@@ -287,18 +335,14 @@ def run_closure(ce, args) -> dict:
                                  "fallback": st["fallback_attempts"] / n},
             "network_quality": quality,
             "training": {"trajectories": len(data.final), "generate_s": data.seconds,
-                         "train_s": ens.seconds, "members": args.members,
-                         "widths": [40, 256, 256, 256, 15], "epochs": args.epochs,
-                         "batch": args.batch, "lr": args.lr,
+                         "train_s": ens.seconds, "members": args.members, "widths": widths,
+                         "epochs": args.epochs, "batch": args.batch, "lr": args.lr,
                          "final_loss": [float(h[-1]) for h in ens.losses]},
         },
         "roofline": {
             "bound": "mfma", "kernel": "gemm_bf16 (ensemble forward, with features and gate)",
-            "achieved": infer_flops / (st["infer_ms"] / 1e3) / 1e12 if st["infer_ms"] else 0.0,
-            "peak": 2500.0, "unit": "TFLOP/s",
-            "frac": (infer_flops / (st["infer_ms"] / 1e3) / 1e12 / 2500.0) if st["infer_ms"] else 0.0,
-            "traffic": None,
-            "flops_per_sim_per_member": mlp_flops},
+            "achieved": infer_tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": infer_tf / 2500.0,
+            "traffic": None, "flops_per_sim_per_member": mlp_flops},
     }
 
 
@@ -307,9 +351,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["ch4", "mixed", "n10", "closure"], default="ch4")
-    ap.add_argument("--train-sims", type=int, default=2_000_000)
-    ap.add_argument("--eval-sims", type=int, default=8_000_000)
+    ap.add_argument("--workload", choices=["ch4", "mixed", "n10", "closure", "closure10"],
+                    default="ch4")
+    ap.add_argument("--train-sims", type=int, default=0, help="closure: 0 = the workload's")
+    ap.add_argument("--eval-sims", type=int, default=0, help="closure: 0 = the workload's")
+    ap.add_argument("--fp64-subset", type=int, default=50_000)
     ap.add_argument("--members", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--batch", type=int, default=8192)
@@ -357,7 +403,7 @@ def main() -> int:
         print("bench.py: no HIP device", file=sys.stderr)
         return 1
 
-    if args.workload == "closure":
+    if args.workload in CLOSURE_SETUPS:
         r = run_closure(ce, args)
         line = {"metric": METRIC, "value": r["within"] / r["closure_s"], "unit": "sims/s",
                 "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

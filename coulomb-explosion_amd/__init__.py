@@ -539,6 +539,27 @@ class MixedBatch:
     def sample(self, seed: int, first_index: int = 0, max_attempts: int = 10000) -> None:
         _check(_lib.ce_engine_sample_species(self._h, seed, first_index, max_attempts))
 
+    @property
+    def _loaded(self) -> int:
+        return self.total
+
+    def enable_state(self, enable: bool = True) -> None:
+        """Keep every sim's integration state at its stop for a later resume."""
+        _check(_lib.ce_engine_enable_state(self._h, int(enable)))
+
+    def state(self):
+        """The saved state per species: dict(pos, vel [count, n, 3], h, e0) (a
+        single dict for a one-species batch)."""
+        n_pos = sum(sp.count * 3 * sp.molecule.size() for sp in self.species)
+        pos, vel = np.zeros(n_pos), np.zeros(n_pos)
+        h, e0 = np.zeros(self.total), np.zeros(self.total)
+        _check(_lib.ce_engine_get_state(self._h, pos.ctypes.data, vel.ctypes.data, h.ctypes.data,
+                                        e0.ctypes.data, self.total))
+        out = [{"pos": p, "vel": v, "h": a, "e0": b} for p, v, a, b in
+               zip(self._split(pos, 3), self._split(vel, 3), self._split_sims(h),
+                   self._split_sims(e0))]
+        return out[0] if len(out) == 1 else out
+
     def set_geometries(self, per_species: list) -> None:
         flat = np.concatenate([np.ascontiguousarray(g, dtype=np.float64).reshape(-1)
                                for g in per_species])
@@ -728,7 +749,7 @@ class Closure:
         _check(_lib.ce_closure_set_members(self._h, arr, len(members)))
         self._members = list(members)  # keep them alive
 
-    def features(self, eng: "BatchedRK45"):
+    def features(self, eng):
         """Raw features [sims, 8n] and prior momenta [sims, n, 3] of the engine's
         saved state."""
         ns = eng._loaded
@@ -737,9 +758,9 @@ class Closure:
         _check(_lib.ce_closure_features(self._h, eng._h, f.ctypes.data, pr.ctypes.data, ns))
         return f, pr
 
-    def run(self, eng: "BatchedRK45", cfg: RunConfig, t_cut: float, gate_tol: float,
-            download: bool = True):
-        if eng._dirty:
+    def run(self, eng, cfg: RunConfig, t_cut: float, gate_tol: float, download: bool = True):
+        """eng: a BatchedRK45 or a one-species MixedBatch (per-sim charges)."""
+        if getattr(eng, "_dirty", False):
             _check(_lib.ce_engine_set_geometries(eng._h, eng._geom.ctypes.data, eng._loaded))
             eng._dirty = False
         c = cfg._c()

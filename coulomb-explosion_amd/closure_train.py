@@ -24,28 +24,47 @@ class ClosureData:
 
 
 def make_data(ce, mol, n_sims, seed, cfg, t_cut, radius, min_sep, rtol, atol,
-              first_index=0, chunk=2_000_000) -> ClosureData:
+              first_index=0, chunk=2_000_000, charge_levels=0,
+              max_total_charge=0) -> ClosureData:
     """Engine-generated (state at t_cut, asymptotic momenta) pairs; only sims
-    stopped by the cut and finished by the continuation are kept."""
+    stopped by the cut and finished by the continuation are kept.  With
+    charge_levels > 0 every sim draws its own charges (uniform in 1..levels)."""
     t0 = time.time()
     clo = ce.Closure(mol)
     n = mol.size()
     feats, priors, finals = [], [], []
     cap = min(chunk, n_sims)
-    eng = ce.BatchedRK45(mol, 1.0, cap, rtol, atol)
-    eng.enable_state()
+    eng = None
     for c0 in range(0, n_sims, cap):
         cnt = min(cap, n_sims - c0)
-        eng.sample(seed, cnt, radius, min_sep, first_index=first_index + c0)
-        cut = eng.run(replace(cfg, t_max=t_cut))
+        if charge_levels:
+            if eng is None or eng.total != cnt:
+                eng = ce.MixedBatch([ce.Species(mol, cnt, radius, min_sep, charge_levels,
+                                                max_total_charge or charge_levels * n)],
+                                    1.0, rtol, atol)
+                eng.enable_state()
+            eng.sample(seed, first_index=first_index + c0)
+        else:
+            if eng is None:
+                eng = ce.BatchedRK45(mol, 1.0, cap, rtol, atol)
+                eng.enable_state()
+            eng.sample(seed, cnt, radius, min_sep, first_index=first_index + c0)
+        cut = _flat(eng.run(replace(cfg, t_max=t_cut)))
         f, pr = clo.features(eng)
-        full = eng.run(replace(cfg, resume=True))
-        keep = (cut.status == ce.SIM_OK_TMAX) & (full.converged == 1)
+        full = _flat(eng.run(replace(cfg, resume=True)))
+        keep = (cut["status"] == ce.SIM_OK_TMAX) & (full["status"] <= ce.SIM_OK_TMAX)
         feats.append(f[keep])
         priors.append(pr[keep])
-        finals.append(full.momenta.reshape(cnt, n, 3)[keep])
+        finals.append(full["momenta"].reshape(cnt, n, 3)[keep])
     return ClosureData(np.concatenate(feats), np.concatenate(priors), np.concatenate(finals),
                        time.time() - t0)
+
+
+def _flat(res) -> dict:
+    """momenta / status of a BatchedRK45 Result or a one-species MixedBatch result."""
+    if isinstance(res, list):
+        return {"momenta": res[0]["momenta"], "status": res[0]["status"]}
+    return {"momenta": res.momenta, "status": res.status}
 
 
 @dataclass

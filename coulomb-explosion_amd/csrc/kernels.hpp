@@ -102,6 +102,17 @@ struct PairK {
 };
 
 // Accelerations a(x) of one lane (x, a: [3N] atom-major), optionally the PE.
+// 1/sqrt(d) in FP64 to ~1 ulp: the hardware estimate refined by two Newton
+// steps y <- y (3/2 - d/2 y^2) (each squares the relative error), for the energy
+// diagnostic (a correctly rounded sqrt and division cost several times more).
+__device__ __forceinline__ double rsqrt_f64(double d) {
+  double y = __builtin_amdgcn_rsq(d);
+  const double hd = 0.5 * d;
+  y = y * fma(-hd * y, y, 1.5);
+  y = y * fma(-hd * y, y, 1.5);
+  return y;
+}
+
 // Pair term: r = x_i - x_j, d2 = r.r (fma), inv = 1/|r|, s = k q_i q_j inv^3,
 // a_i += r s and a_j -= r s as single-rounding FMAs (Newton 3), then a_i *= 1/m_i.
 template <class T, int N, bool PS, bool WITH_PE, bool WITH_IM = true>
@@ -386,7 +397,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
           const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
           const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
-          pe_end += pk.template getd<i, j>() / __builtin_sqrt(rx * rx + ry * ry + rz * rz);
+          pe_end = fma(pk.template getd<i, j>(), rsqrt_f64(rx * rx + ry * ry + rz * rz), pe_end);
         });
       });
       const uint64_t base = sg.pos_off + loc * N3;

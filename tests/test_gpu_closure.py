@@ -167,3 +167,41 @@ def test_closure_gate_follows_spread(gpu, trained):
     np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
     assert clo.stats()["fallback"] == np.count_nonzero(~acc)
     assert np.all(res.status[~acc] == ce.SIM_OK_TMAX) and np.all(res.t_final[~acc] == 1e4)
+
+
+def test_closure_per_sim_charges_n10(gpu):
+    """The N=10 mixed-charge path (one-species MixedBatch): features and prior
+    use each sim's charges; gate 0 equals cut + resume; a wide-open gate takes
+    the (untrained) ensemble everywhere."""
+    ce = gpu
+    C, O, H = 21874.66, 29156.95, 1837.15
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in [C, C, O, O] + [H] * 6])
+    cfg = ce.RunConfig(**RULE)
+
+    def engine():
+        e = ce.MixedBatch([ce.Species(mol, 1500, 3.5, 0.9, charge_levels=3)], 1.0, 1e-6, 1e-9)
+        e.sample(2024)
+        e.enable_state()
+        return e
+
+    eng = engine()
+    eng.run(replace(cfg, t_max=T_CUT))
+    st = eng.state()
+    q = eng.charges()[0]
+    assert len(np.unique(q)) == 3
+    clo = ce.Closure(mol)
+    f, pr = clo.features(eng)
+    m = mol.masses
+    for s in range(0, 1500, 101):
+        x, v = st["pos"][s], st["vel"][s]
+        np.testing.assert_allclose(pr[s], closure_ref.prior(x, v, m, q[s], st["e0"][s]),
+                                   rtol=1e-12, atol=1e-12 * np.abs(pr[s]).max())
+        np.testing.assert_allclose(f[s], closure_ref.features(x, v, m, q[s]), rtol=2e-7, atol=1e-6)
+    resumed = eng.run(replace(cfg, resume=True))[0]
+    nets = [ce.MLP([80, 64, 30], seed=k) for k in range(2)]
+    clo.set_normalization(f.mean(0), np.maximum(f.std(0), 1e-6), np.ones(30, np.float32))
+    clo.set_members(nets)
+    res0 = clo.run(engine(), cfg, T_CUT, 0.0)[0]
+    np.testing.assert_array_equal(res0["momenta"], resumed["momenta"])
+    res1 = clo.run(engine(), cfg, T_CUT, 1e30)[0]
+    assert np.all(res1["status"] == ce.SIM_CLOSURE) and clo.stats()["fallback"] == 0

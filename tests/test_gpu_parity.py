@@ -116,6 +116,10 @@ def test_f64_engine_matches_oracle_lane_f64(gpu, oracle):
                               ffi.make_config(rtol=1e-10, atol=1e-14, **rule), precision="f64")
         np.testing.assert_array_equal(res.steps, ref.steps)
         assert per_atom_rel(res.momenta, ref.momenta).max() < 1e-12
+        # FP64 energy diagnostic (KE + PE - E0)/E0 at the stop: the engine's
+        # refined-rsqrt PE against the oracle's correctly rounded one
+        np.testing.assert_allclose(res.energy_err, ref.energy_err, rtol=0, atol=1e-13)
+        assert np.abs(ref.energy_err).max() < 1e-8
 
 
 def test_pe_rule_reference_defaults_vs_ref_engine(gpu, oracle):
