@@ -144,6 +144,7 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_mlp_forward": ([P, P, S, P], C.c_int),
         "ce_mlp_gradients": ([P, P, P, S, P, P], C.c_int),
         "ce_mlp_adam_step": ([P, C.c_float], C.c_int),
+        "ce_mlp_set_fused": ([P, C.c_int32], C.c_int),
         "ce_mlp_set_dataset": ([P, P, P, S], C.c_int),
         "ce_mlp_train_epoch": ([P, S, C.c_float, C.c_uint64, P], C.c_int),
         "ce_closure_create": ([C.c_int32, P, P, C.c_double, P], C.c_int),
@@ -697,6 +698,14 @@ class MLP:
         _check(_lib.ce_mlp_gradients(self._h, x.ctypes.data, t.ctypes.data, x.shape[0],
                                      g.ctypes.data, C.byref(loss)))
         return float(loss.value), g
+
+    def set_fused(self, enable: bool = True) -> bool:
+        """Inference through the fused whole-network kernel where the shape allows;
+        returns whether it is in use."""
+        rc = _lib.ce_mlp_set_fused(self._h, int(enable))
+        if rc < 0:
+            _check(rc)
+        return rc == 1
 
     def adam_step(self, lr: float) -> None:
         _check(_lib.ce_mlp_adam_step(self._h, lr))

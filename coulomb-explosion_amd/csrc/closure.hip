@@ -57,81 +57,95 @@ struct FeatureArgs {
   double* prior;          // [count][3n]
 };
 
+constexpr int kFeatBlock = 128;  // sims per block of the feature kernel
+
+// One thread per sim; the rows are staged in LDS and written back coalesced.
 template <int N>
-__global__ void closure_features(const FeatureArgs a, const ClosureConst cc) {
-  const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (r >= a.count) return;
+__global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs a,
+                                                               const ClosureConst cc) {
+  __shared__ float fs[kFeatBlock][8 * N];
+  __shared__ double ps[kFeatBlock][3 * N];
+  const uint64_t r0 = blockIdx.x * uint64_t(kFeatBlock);
+  const uint64_t r = r0 + threadIdx.x;
   const uint64_t g = a.first + r;
-  if (!a.all && a.status[g] != ce::kSimOkTmax) return;
-  double x[3 * N], v[3 * N], q[N], p[3 * N];
-  double cm[3] = {0.0, 0.0, 0.0}, mt = 0.0;
+  if (r < a.count && (a.all || a.status[g] == ce::kSimOkTmax)) {
+    double x[3 * N], v[3 * N], q[N], p[3 * N];
+    double cm[3] = {0.0, 0.0, 0.0}, mt = 0.0;
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    q[i] = a.qsim != nullptr ? static_cast<double>(a.qsim[g * N + i]) : cc.q[i];
-    mt += cc.m[i];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      x[3 * i + c] = a.x[g * 3 * N + 3 * i + c];
-      v[3 * i + c] = a.v[g * 3 * N + 3 * i + c];
-      p[3 * i + c] = cc.m[i] * v[3 * i + c];
-      cm[c] += cc.m[i] * x[3 * i + c];
-    }
-  }
-  // prior: straight-line impulse of every pair, then KE rescaled to E0
-  double pp[3 * N];
-#pragma unroll
-  for (int j = 0; j < 3 * N; ++j) pp[j] = p[j];
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-#pragma unroll
-    for (int j = i + 1; j < N; ++j) {
-      double rr[3], uu[3];
-      double c2 = 0.0, a2 = 0.0, b = 0.0;
+    for (int i = 0; i < N; ++i) {
+      q[i] = a.qsim != nullptr ? static_cast<double>(a.qsim[g * N + i]) : cc.q[i];
+      mt += cc.m[i];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        rr[c] = x[3 * i + c] - x[3 * j + c];
-        uu[c] = v[3 * i + c] - v[3 * j + c];
-        c2 += rr[c] * rr[c];
-        a2 += uu[c] * uu[c];
-        b += rr[c] * uu[c];
+        x[3 * i + c] = a.x[g * 3 * N + 3 * i + c];
+        v[3 * i + c] = a.v[g * 3 * N + 3 * i + c];
+        p[3 * i + c] = cc.m[i] * v[3 * i + c];
+        cm[c] += cc.m[i] * x[3 * i + c];
       }
-      const double rn = sqrt(c2), un = sqrt(a2);
-      const double w = rn * un + b;
-      if (!(rn > 0.0) || !(un > 0.0) || !(w > 1e-300)) continue;
-      const double s = cc.k * q[i] * q[j] / w;
+    }
+    // prior: straight-line impulse of every pair, then KE rescaled to E0
+    double pp[3 * N];
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) pp[j] = p[j];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) {
+        double rr[3], uu[3];
+        double c2 = 0.0, a2 = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          rr[c] = x[3 * i + c] - x[3 * j + c];
+          uu[c] = v[3 * i + c] - v[3 * j + c];
+          c2 += rr[c] * rr[c];
+          a2 += uu[c] * uu[c];
+          b += rr[c] * uu[c];
+        }
+        const double rn = sqrt(c2), un = sqrt(a2);
+        const double w = rn * un + b;
+        if (!(rn > 0.0) || !(un > 0.0) || !(w > 1e-300)) continue;
+        const double s = cc.k * q[i] * q[j] / w;
+        const double sr = s / rn, su = s / un;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double J = sr * rr[c] + su * uu[c];
+          pp[3 * i + c] += J;
+          pp[3 * j + c] -= J;
+        }
+      }
+    }
+    double ke = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      ke += 0.5 * (pp[3 * i] * pp[3 * i] + pp[3 * i + 1] * pp[3 * i + 1] +
+                   pp[3 * i + 2] * pp[3 * i + 2]) / cc.m[i];
+    const double sc = (ke > 0.0 && a.e0[g] > 0.0) ? sqrt(a.e0[g] / ke) : 1.0;
+#pragma unroll
+    for (int j = 0; j < 3 * N; ++j) ps[threadIdx.x][j] = pp[j] * sc;
+    // features
+    float f[8 * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double J = s * (rr[c] / rn + uu[c] / un);
-        pp[3 * i + c] += J;
-        pp[3 * j + c] -= J;
+        f[6 * i + c] = static_cast<float>(x[3 * i + c] - cm[c] / mt);
+        f[6 * i + 3 + c] = static_cast<float>(p[3 * i + c]);
       }
+      f[6 * N + i] = static_cast<float>(q[i]);
+      f[7 * N + i] = static_cast<float>(cc.m[i]);
     }
+#pragma unroll
+    for (int j = 0; j < 8 * N; ++j)
+      fs[threadIdx.x][j] = a.shift != nullptr ? (f[j] - a.shift[j]) * a.inv_scale[j] : f[j];
   }
-  double ke = 0.0;
-#pragma unroll
-  for (int i = 0; i < N; ++i)
-    ke += 0.5 * (pp[3 * i] * pp[3 * i] + pp[3 * i + 1] * pp[3 * i + 1] +
-                 pp[3 * i + 2] * pp[3 * i + 2]) / cc.m[i];
-  const double sc = (ke > 0.0 && a.e0[g] > 0.0) ? sqrt(a.e0[g] / ke) : 1.0;
-  double* pr = a.prior + r * 3 * N;
-#pragma unroll
-  for (int j = 0; j < 3 * N; ++j) pr[j] = pp[j] * sc;
-  // features
-  float f[8 * N];
-#pragma unroll
-  for (int i = 0; i < N; ++i) {
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      f[6 * i + c] = static_cast<float>(x[3 * i + c] - cm[c] / mt);
-      f[6 * i + 3 + c] = static_cast<float>(p[3 * i + c]);
-    }
-    f[6 * N + i] = static_cast<float>(q[i]);
-    f[7 * N + i] = static_cast<float>(cc.m[i]);
-  }
-  float* fo = a.feat + r * 8 * N;
-#pragma unroll
-  for (int j = 0; j < 8 * N; ++j)
-    fo[j] = a.shift != nullptr ? (f[j] - a.shift[j]) * a.inv_scale[j] : f[j];
+  __syncthreads();
+  // the block's rows are contiguous in feat / prior (rows not computed carry
+  // whatever LDS held: nothing reads them)
+  const uint64_t rows = a.count - r0 < kFeatBlock ? a.count - r0 : kFeatBlock;
+  const float* fsf = &fs[0][0];
+  for (uint64_t e = threadIdx.x; e < rows * 8 * N; e += kFeatBlock) a.feat[r0 * 8 * N + e] = fsf[e];
+  const double* psf = &ps[0][0];
+  for (uint64_t e = threadIdx.x; e < rows * 3 * N; e += kFeatBlock) a.prior[r0 * 3 * N + e] = psf[e];
 }
 
 struct GateArgs {
@@ -148,50 +162,72 @@ struct GateArgs {
   uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut
 };
 
+// Wave-aggregated append: one atomic per wave, each flagged lane gets its slot.
+__device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t* counter,
+                                            uint32_t* list) {
+  const uint64_t mask = __ballot(flag);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == __ffsll(static_cast<long long>(mask)) - 1)
+    base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, __ffsll(static_cast<long long>(mask)) - 1);
+  if (flag) list[base + __popcll(mask & ((1ull << lane) - 1))] = value;
+}
+
 template <int N>
-__global__ void closure_gate(const GateArgs a) {
+__global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  if (r >= a.count) return;
   const uint64_t g = a.first + r;
-  if (a.status[g] != ce::kSimOkTmax) return;
-  atomicAdd(a.counts + 1, 1u);
-  const uint64_t cs = a.count * 3 * N;  // member stride
-  bool ok = a.members >= 2;
-  double pm[3 * N];
+  const bool cand = r < a.count && a.status[g] == ce::kSimOkTmax;
+  bool ok = false;
+  if (cand) {
+    // member-major passes: the 3N loads of a member are independent of each other
+    const uint64_t cs = a.count * 3 * N;  // member stride
+    const float* yr = a.y + r * 3 * N;
+    const double inv_m = a.members > 0 ? 1.0 / a.members : 0.0;
+    const double inv_m1 = a.members > 1 ? 1.0 / (a.members - 1) : 1.0;
+    double pm[3 * N];
 #pragma unroll
-  for (int j = 0; j < 3 * N; ++j) {
-    double s = 0.0;
-    for (int k = 0; k < a.members; ++k) s += a.y[k * cs + r * 3 * N + j];
-    pm[j] = a.members > 0 ? s / a.members : 0.0;
-  }
+    for (int j = 0; j < 3 * N; ++j) pm[j] = 0.0;
+    for (int k = 0; k < a.members; ++k)
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    double var = 0.0, p2 = 0.0;
+      for (int j = 0; j < 3 * N; ++j) pm[j] += yr[k * cs + j];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int j = 3 * i + c;
-      for (int k = 0; k < a.members; ++k) {
-        const double d = (a.y[k * cs + r * 3 * N + j] - pm[j]) * a.out_scale[j];
-        var += d * d;
+    for (int j = 0; j < 3 * N; ++j) pm[j] *= inv_m;
+    double var[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) var[i] = 0.0;
+    for (int k = 0; k < a.members; ++k)
+#pragma unroll
+      for (int j = 0; j < 3 * N; ++j) {
+        const double d = (yr[k * cs + j] - pm[j]) * a.out_scale[j];
+        var[j / 3] += d * d;
       }
-      const double pj = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];
-      p2 += pj * pj;
-    }
-    var /= (a.members > 1 ? a.members - 1 : 1);
-    ok = ok && (var <= a.tol2 * p2);  // false for NaN
-  }
-  if (ok) {
+    ok = a.members >= 2;
 #pragma unroll
-    for (int j = 0; j < 3 * N; ++j) {
-      const double pj = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];
-      a.mom[g * 3 * N + j] = pj;
-      a.mom32[g * 3 * N + j] = static_cast<float>(pj);
+    for (int i = 0; i < N; ++i) {
+      double p2 = 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int j = 3 * i + c;
+        pm[j] = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];  // predicted momentum
+        p2 += pm[j] * pm[j];
+      }
+      ok = ok && (var[i] * inv_m1 <= a.tol2 * p2);  // false for NaN
     }
-    a.status[g] = ce::kSimClosure;
-  } else {
-    const uint32_t slot = atomicAdd(a.counts, 1u);
-    a.list[slot] = static_cast<uint32_t>(g);
+    if (ok) {
+#pragma unroll
+      for (int j = 0; j < 3 * N; ++j) {
+        a.mom[g * 3 * N + j] = pm[j];
+        a.mom32[g * 3 * N + j] = static_cast<float>(pm[j]);
+      }
+      a.status[g] = ce::kSimClosure;
+    }
   }
+  const uint64_t cmask = __ballot(cand);
+  if (__lane_id() == 0 && cmask != 0) atomicAdd(a.counts + 1, static_cast<uint32_t>(__popcll(cmask)));
+  wave_append(cand && !ok, static_cast<uint32_t>(g), a.counts, a.list);
 }
 
 __global__ void zero_counts(uint32_t* p) {
@@ -269,7 +305,9 @@ int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
   switch (c->n) {
 #define CL_F(NN)                                                                      \
   case NN:                                                                            \
-    hipLaunchKernelGGL(closure_features<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a, \
+    hipLaunchKernelGGL(closure_features<NN>,                                          \
+                       dim3(static_cast<unsigned>((a.count + kFeatBlock - 1) / kFeatBlock)),  \
+                       dim3(kFeatBlock), 0, s, a,                                         \
                        c->cc);                                                        \
     break;
     CL_F(1) CL_F(2) CL_F(3) CL_F(4) CL_F(5) CL_F(6) CL_F(7) CL_F(8) CL_F(9) CL_F(10)

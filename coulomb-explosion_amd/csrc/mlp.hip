@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "coulomb_explosion.h"
@@ -29,14 +30,26 @@ int set_error(int code, const std::string& msg);
 
 namespace {
 
+// Compile-time loop: f(std::integral_constant<int, i>) for i in [B, E).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
+// Round to nearest even with the gfx950 conversion instruction (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
-  // round to nearest even (inputs are finite in this network)
-  uint32_t u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
+  return __builtin_bit_cast(uint16_t, static_cast<__bf16>(f));
+}
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pack_bf16(float lo, float hi) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{lo, hi}, bf16x2));
 }
 // sigmoid with the hardware exp2 and reciprocal (about 1 ulp each; the result is
 // rounded to bf16 or feeds a bf16 product)
@@ -285,6 +298,214 @@ __global__ void adam(float* p, const float* g, float* m, float* v, int64_t n, fl
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+
+// ---------------------------------------------------------- fused forward ----
+// Inference of a whole network [w0, H, ..., H, w_out] (H = 256 with w0 <= 64, or
+// H = 512 with w0 <= 128; w_out <= 32) in one kernel, computed transposed:
+// C^T[n][m] = sum_k W[n][k] act[m][k], so the weights are the MFMA A operand,
+// read from LDS, and the activations are the B operand, held in registers for
+// the whole network.  A workgroup owns 4 waves x 32 SB samples; the weights
+// stream through LDS in tiles of H rows x KC columns (32 KB), double-buffered
+// and shared by the 4 waves, the next tile's global loads in flight while the
+// current one feeds the MFMAs.  Between layers each lane turns its accumulators
+// (features 8q + 4h + t of a 32-feature block, h = lane / 32) into the next
+// layer's B fragments (8 consecutive features per lane) with one
+// v_permlane32_swap per dword pair.  Products and their K order are those of
+// the layer-by-layer GEMM.
+constexpr int kFusedMaxLayers = 8;
+
+struct FusedArgs {
+  const float* x;  // [rows][w0] fp32
+  int64_t rows;
+  int w0;
+  int layers;      // weight layers (>= 2): w0->H, (layers - 2) x H->H, H->w_out
+  int w_out;
+  const uint16_t* W[kFusedMaxLayers];  // padded bf16 [round32(out)][round64(in)]
+  const float* b[kFusedMaxLayers];     // [out]
+  float* y;                            // [rows][w_out]
+};
+
+template <int H>
+struct FusedTiles {
+  static constexpr int KC = 16384 / H;  // k columns per tile (32 KB of bf16)
+  static constexpr int TSTR = KC + 8;   // LDS row stride in elements
+  static constexpr int PER = H / KC;    // tiles of one H-deep layer
+};
+
+// Loads tile t of the layer stream into registers: 8 x 16 bytes per thread.
+template <int H, int K0C>
+__device__ __forceinline__ void fused_fetch(const FusedArgs& a, int t, uint4 (&r)[8]) {
+  using FT = FusedTiles<H>;
+  int layer, chunk;
+  if (t < K0C) {
+    layer = 0;
+    chunk = t;
+  } else {
+    layer = 1 + (t - K0C) / FT::PER;
+    chunk = (t - K0C) % FT::PER;
+  }
+  const int rows = layer == a.layers - 1 ? 32 : H;
+  const int ldw = layer == 0 ? K0C * FT::KC : H;
+  const uint16_t* W = a.W[layer] + chunk * FT::KC;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int e = threadIdx.x + 256 * i;  // 8-element chunk id
+    const int row = e / (FT::KC / 8), c8 = e % (FT::KC / 8);
+    if (row < rows) r[i] = *reinterpret_cast<const uint4*>(W + row * ldw + 8 * c8);
+  }
+}
+
+template <int H>
+__device__ __forceinline__ void fused_stage(uint16_t (*T)[FusedTiles<H>::TSTR], int rows,
+                                            const uint4 (&r)[8]) {
+  using FT = FusedTiles<H>;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    const int row = e / (FT::KC / 8), c8 = e % (FT::KC / 8);
+    if (row < rows) *reinterpret_cast<uint4*>(&T[row][8 * c8]) = r[i];
+  }
+}
+
+// One layer over the tile stream: acc[nb][sb] += W-tile x act for the layer's
+// PER_L tiles (t advances).  NB = output blocks of 32 (H / 32, or 1 for the
+// output layer); KSL = k16 steps of this layer's input.
+template <int H, int K0C, int SB, int NB, int PER_L>
+__device__ __forceinline__ void fused_layer_t(const FusedArgs& a,
+                                              uint16_t (*tile)[H][FusedTiles<H>::TSTR], int& t,
+                                              int total, const bf16x8 (&act)[H / 16][SB],
+                                              f32x16 (&acc)[NB][SB]) {
+  using FT = FusedTiles<H>;
+  const int lane = threadIdx.x & 63, ln = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+#pragma unroll
+    for (int j = 0; j < SB; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+  static_for<0, PER_L>([&](auto C) {
+    constexpr int c = decltype(C)::value;
+    __syncthreads();  // tile t staged by every thread; the other buffer is free
+    uint4 pre[8];
+    const bool more = t + 1 < total;
+    if (more) fused_fetch<H, K0C>(a, t + 1, pre);
+    const uint16_t (*T)[FT::TSTR] = tile[t & 1];
+    static_for<0, FT::KC / 16>([&](auto KK) {
+      constexpr int kk = decltype(KK)::value;
+      constexpr int ks = c * (FT::KC / 16) + kk;
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb) {
+        const bf16x8 w = *reinterpret_cast<const bf16x8*>(&T[32 * nb + ln][16 * kk + 8 * h]);
+#pragma unroll
+        for (int sb = 0; sb < SB; ++sb)
+          acc[nb][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, act[ks][sb], acc[nb][sb], 0, 0, 0);
+      }
+    });
+    if (more) {
+      int layer = t + 1 < K0C ? 0 : 1 + (t + 1 - K0C) / FT::PER;
+      fused_stage<H>(tile[(t + 1) & 1], layer == a.layers - 1 ? 32 : H, pre);
+    }
+    ++t;
+  });
+}
+
+template <int H, int SB>
+__device__ __forceinline__ void fused_hidden_epilogue(const float* bias,
+                                                      const f32x16 (&acc)[H / 32][SB],
+                                                      bf16x8 (&act)[H / 16][SB]) {
+  const int h = (threadIdx.x & 63) >> 5;
+#pragma unroll
+  for (int nb = 0; nb < H / 32; ++nb) {
+    float bv[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) bv[4 * q + u] = bias[32 * nb + 8 * q + 4 * h + u];
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb) {
+      float v[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) v[r] = silu(acc[nb][sb][r] + bv[r]);
+      // lane holds features 8q + 4h + t (q = r / 4, t = r % 4) of this block
+      uint32_t P0 = pack_bf16(v[0], v[1]), P1 = pack_bf16(v[2], v[3]);
+      uint32_t Q0 = pack_bf16(v[4], v[5]), Q1 = pack_bf16(v[6], v[7]);
+      uint32_t R0 = pack_bf16(v[8], v[9]), R1 = pack_bf16(v[10], v[11]);
+      uint32_t S0 = pack_bf16(v[12], v[13]), S1 = pack_bf16(v[14], v[15]);
+      // lanes 0-31 take lanes 32-63's (P, R); lanes 32-63 take lanes 0-31's (Q, S)
+      auto x0 = __builtin_amdgcn_permlane32_swap(P0, Q0, false, false);
+      auto x1 = __builtin_amdgcn_permlane32_swap(P1, Q1, false, false);
+      auto x2 = __builtin_amdgcn_permlane32_swap(R0, S0, false, false);
+      auto x3 = __builtin_amdgcn_permlane32_swap(R1, S1, false, false);
+      const uint32_t lo[4] = {x0[0], x1[0], x0[1], x1[1]};
+      const uint32_t hi[4] = {x2[0], x3[0], x2[1], x3[1]};
+      act[2 * nb][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(lo));
+      act[2 * nb + 1][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(hi));
+    }
+  }
+}
+
+template <int H, int SB, int K0C>
+__global__ void __launch_bounds__(256) mlp_fused_forward(const FusedArgs a) {
+  using FT = FusedTiles<H>;
+  __shared__ __align__(16) uint16_t tile[2][H][FT::TSTR];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
+  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (4 * 32 * SB) + wave * 32 * SB;
+  // inputs: lane holds sample m0 + 32 sb + ln, features 16 ks + 8 h + 0..7
+  bf16x8 act[H / 16][SB];
+#pragma unroll
+  for (int ks = 0; ks < H / 16; ++ks)
+#pragma unroll
+    for (int sb = 0; sb < SB; ++sb) {
+      uint32_t d[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float f[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int k = 16 * ks + 8 * h + 2 * j + u;
+          const int64_t m = m0 + 32 * sb + ln;
+          f[u] = (ks < K0C * FT::KC / 16 && k < a.w0 && m < a.rows) ? a.x[m * a.w0 + k] : 0.0f;
+        }
+        d[j] = pack_bf16(f[0], f[1]);
+      }
+      act[ks][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(d));
+    }
+  const int total = K0C + (a.layers - 1) * FT::PER;
+  int t = 0;
+  {
+    uint4 pre[8];
+    fused_fetch<H, K0C>(a, 0, pre);
+    fused_stage<H>(tile[0], H, pre);
+  }
+  f32x16 acc[H / 32][SB];
+  fused_layer_t<H, K0C, SB, H / 32, K0C>(a, tile, t, total, act, acc);
+  fused_hidden_epilogue<H, SB>(a.b[0], acc, act);
+  for (int l = 1; l + 1 < a.layers; ++l) {
+    fused_layer_t<H, K0C, SB, H / 32, FT::PER>(a, tile, t, total, act, acc);
+    fused_hidden_epilogue<H, SB>(a.b[l], acc, act);
+  }
+  f32x16 out[1][SB];
+  fused_layer_t<H, K0C, SB, 1, FT::PER>(a, tile, t, total, act, out);
+  const float* bias = a.b[a.layers - 1];
+#pragma unroll
+  for (int sb = 0; sb < SB; ++sb) {
+    const int64_t m = m0 + 32 * sb + ln;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (n < a.w_out && m < a.rows) a.y[m * a.w_out + n] = out[0][sb][r] + bias[n];
+    }
+  }
+}
+
+// Padded bf16 copy [round32(out)][round64(in)] of an fp32 [out][in] matrix.
+__global__ void pad_bf16(const float* w, int out, int in, int out_p, int in_p, uint16_t* dst) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= int64_t(out_p) * in_p) return;
+  const int n = static_cast<int>(i / in_p), k = static_cast<int>(i % in_p);
+  dst[i] = (n < out && k < in) ? f32_to_bf16_bits(w[int64_t(n) * in + k]) : uint16_t(0);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ host ----
@@ -298,6 +519,9 @@ struct ce_mlp {
   int64_t n_params = 0;
   float *P = nullptr, *G = nullptr, *M1 = nullptr, *M2 = nullptr;  // master, grad, Adam moments
   std::vector<uint16_t*> Wb;  // bf16 copies [out][in]
+  int fused_h = 0;            // hidden width of the fused inference kernel, 0 = none
+  bool use_fused = true;      // inference (ce_mlp_forward, closure) through it
+  std::vector<uint16_t*> Wp;  // padded bf16 copies for the fused kernel
   int64_t adam_t = 0;
   // batch workspace
   int64_t cap = 0;
@@ -344,6 +568,7 @@ void free_mlp(ce_mlp* m) {
   f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16);
   f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
   for (auto* p : m->Wb) f(p);
+  for (auto* p : m->Wp) f(p);
   for (auto* p : m->act) f(p);
   for (auto* p : m->Z) f(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -403,12 +628,19 @@ int gemm(hipStream_t s, GemmArgs g, bool split_k = false) {
   return CE_OK;
 }
 
+int round_up(int v, int m) { return (v + m - 1) / m * m; }
+
 int export_all(ce_mlp* m) {
   for (int l = 0; l < m->L; ++l) {
     const int in = m->widths[l], out = m->widths[l + 1];
     const int64_t n = static_cast<int64_t>(in) * out;
     hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, m->stream, m->P + m->w_off[l],
                        m->Wb[l], n);
+    if (m->fused_h) {
+      const int op = round_up(out, 32), ip = round_up(in, 64);
+      hipLaunchKernelGGL(pad_bf16, dim3(blocks_for(int64_t(op) * ip)), dim3(256), 0, m->stream,
+                         m->P + m->w_off[l], out, in, op, ip, m->Wp[l]);
+    }
   }
   MLP_HIP(hipGetLastError());
   return CE_OK;
@@ -557,6 +789,21 @@ int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp
   m->Wb.assign(m->L, nullptr);
   for (int l = 0; l < m->L && rc == CE_OK; ++l)
     rc = dalloc(&m->Wb[l], static_cast<int64_t>(widths[l]) * widths[l + 1]);
+  // the fused inference kernel covers [w0, H, ..., H, w_out <= 32] with H = 256 and
+  // w0 <= 64, or H = 512 and w0 <= 128
+  {
+    const int h = m->L >= 2 ? widths[1] : 0;
+    bool ok = ((h == 256 && widths[0] <= 64) || (h == 512 && widths[0] <= 128)) &&
+              widths[m->L] <= 32 && m->L <= kFusedMaxLayers;
+    for (int l = 1; l < m->L && ok; ++l) ok = widths[l] == h;
+    m->fused_h = ok ? h : 0;
+  }
+  if (m->fused_h) {
+    m->Wp.assign(m->L, nullptr);
+    for (int l = 0; l < m->L && rc == CE_OK; ++l)
+      rc = dalloc(&m->Wp[l], static_cast<int64_t>(round_up(widths[l + 1], 32)) *
+                                 round_up(widths[l], 64));
+  }
   if (rc == CE_OK) {
     std::vector<float> p;
     init_params(m, seed, p);
@@ -615,7 +862,8 @@ int ce_mlp_forward(ce_mlp* m, const float* x_host, size_t batch, float* y_host) 
     if (rc != CE_OK) return rc;
     MLP_HIP(hipMemcpyAsync(m->xin, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
                            hipMemcpyHostToDevice, m->stream));
-    rc = forward(m, m->xin, B, false, m->out, m->stream);
+    rc = (m->fused_h && m->use_fused) ? ce::mlp_forward_device(m, m->xin, B, m->out, m->stream)
+                                      : forward(m, m->xin, B, false, m->out, m->stream);
     if (rc != CE_OK) return rc;
     MLP_HIP(hipMemcpyAsync(y_host + b0 * m->widths[m->L], m->out,
                            B * m->widths[m->L] * sizeof(float), hipMemcpyDeviceToHost, m->stream));
@@ -748,6 +996,14 @@ int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed,
   return CE_OK;
 }
 
+// Inference through the fused whole-network kernel when the shape allows it
+// (default) or layer by layer; returns 1 when the fused kernel is used.
+int ce_mlp_set_fused(ce_mlp* m, int32_t enable) {
+  if (!m) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  m->use_fused = enable != 0;
+  return (m->fused_h && m->use_fused) ? 1 : 0;
+}
+
 // One Adam step on the gradients left by ce_mlp_gradients (tests).
 int ce_mlp_adam_step(ce_mlp* m, float lr) {
   if (!m) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -768,6 +1024,28 @@ int mlp_width(const ce_mlp* m, int layer) {
 int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 
 int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
+  if (m->fused_h && m->use_fused && B > 0) {
+    FusedArgs a{};
+    a.x = x;
+    a.rows = B;
+    a.w0 = m->widths[0];
+    a.layers = m->L;
+    a.w_out = m->widths[m->L];
+    for (int l = 0; l < m->L; ++l) {
+      a.W[l] = m->Wp[l];
+      a.b[l] = m->P + m->b_off[l];
+    }
+    a.y = y;
+    if (m->fused_h == 256) {  // 2 x 32 samples per wave, inputs in one 64-wide tile
+      const unsigned grid = static_cast<unsigned>((B + 255) / 256);
+      hipLaunchKernelGGL((mlp_fused_forward<256, 2, 1>), dim3(grid), dim3(256), 0, s, a);
+    } else {  // 32 samples per wave, inputs in four 32-wide tiles
+      const unsigned grid = static_cast<unsigned>((B + 127) / 128);
+      hipLaunchKernelGGL((mlp_fused_forward<512, 1, 4>), dim3(grid), dim3(256), 0, s, a);
+    }
+    MLP_HIP(hipGetLastError());
+    return CE_OK;
+  }
   const int64_t chunk = 262144;
   const int64_t c = B < chunk ? B : chunk;
   int rc = ensure_batch(m, c);

@@ -265,6 +265,11 @@ int ce_mlp_get_params(ce_mlp* m, float* params);
 int ce_mlp_set_params(ce_mlp* m, const float* params);  /* also resets Adam */
 /* y [batch][w_last] = net(x [batch][w0]); host buffers. */
 int ce_mlp_forward(ce_mlp* m, const float* x, size_t batch, float* y);
+/* Inference (ce_mlp_forward and the closure) runs networks shaped
+ * [w0 <= 96, H, ..., H, w_out <= 32] with H = 256 or 512 in one fused kernel
+ * (activations kept on chip; same bits as the layer-by-layer GEMMs) unless
+ * disabled here.  Returns 1 if the fused kernel is in use, 0 if not, < 0 on error. */
+int ce_mlp_set_fused(ce_mlp* m, int32_t enable);
 /* Loss and d loss / d params for one batch (host buffers). */
 int ce_mlp_gradients(ce_mlp* m, const float* x, const float* t, size_t batch, float* grad,
                      float* loss);

@@ -142,3 +142,29 @@ def test_mlp_argument_errors(gpu):
         net.forward(np.zeros((3, 5), np.float32))
     with pytest.raises(RuntimeError):
         net.train_epoch(16, 1e-3)  # no training set
+
+
+@pytest.mark.parametrize("widths,rows", [([40, 256, 256, 256, 15], 1000),
+                                         ([80, 512, 512, 512, 512, 30], 300),
+                                         ([13, 256, 256, 6], 257)])
+def test_fused_forward_same_bits_as_layer_by_layer(gpu, widths, rows):
+    """The fused inference kernel accumulates in the GEMM's K order: same bits."""
+    ce = gpu
+    rng = np.random.default_rng(rows)
+    net = ce.MLP(widths, seed=9)
+    p = mlp_ref.glorot_params(widths, rng).astype(np.float32)
+    net.set_params(p)
+    x = rng.normal(size=(rows, widths[0])).astype(np.float32)
+    assert net.set_fused(True)
+    y_fused = net.forward(x)
+    assert not net.set_fused(False)
+    y_layers = net.forward(x)
+    np.testing.assert_array_equal(y_fused, y_layers)
+    yb, _ = mlp_ref.forward(p, widths, x, exact=False)
+    assert np.abs(y_fused - yb).max() < 1e-2 * np.abs(yb).max()
+
+
+def test_fused_not_used_for_other_shapes(gpu):
+    ce = gpu
+    assert not ce.MLP([7, 64, 50, 3]).set_fused(True)
+    assert not ce.MLP([100, 256, 256, 8]).set_fused(True)  # input wider than 96
