@@ -521,6 +521,7 @@ struct ce_mlp {
   std::vector<uint16_t*> Wb;  // bf16 copies [out][in]
   int fused_h = 0;            // hidden width of the fused inference kernel, 0 = none
   bool use_fused = true;      // inference (ce_mlp_forward, closure) through it
+  bool wp_stale = true;       // Wp older than the master parameters
   std::vector<uint16_t*> Wp;  // padded bf16 copies for the fused kernel
   int64_t adam_t = 0;
   // batch workspace
@@ -636,12 +637,8 @@ int export_all(ce_mlp* m) {
     const int64_t n = static_cast<int64_t>(in) * out;
     hipLaunchKernelGGL(to_bf16, dim3(blocks_for(n)), dim3(256), 0, m->stream, m->P + m->w_off[l],
                        m->Wb[l], n);
-    if (m->fused_h) {
-      const int op = round_up(out, 32), ip = round_up(in, 64);
-      hipLaunchKernelGGL(pad_bf16, dim3(blocks_for(int64_t(op) * ip)), dim3(256), 0, m->stream,
-                         m->P + m->w_off[l], out, in, op, ip, m->Wp[l]);
-    }
   }
+  m->wp_stale = true;  // the padded copies are rebuilt before the next fused inference
   MLP_HIP(hipGetLastError());
   return CE_OK;
 }
@@ -1025,6 +1022,15 @@ int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 
 int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
   if (m->fused_h && m->use_fused && B > 0) {
+    if (m->wp_stale) {
+      for (int l = 0; l < m->L; ++l) {
+        const int in = m->widths[l], out = m->widths[l + 1];
+        const int op = round_up(out, 32), ip = round_up(in, 64);
+        hipLaunchKernelGGL(pad_bf16, dim3(blocks_for(int64_t(op) * ip)), dim3(256), 0, s,
+                           m->P + m->w_off[l], out, in, op, ip, m->Wp[l]);
+      }
+      m->wp_stale = false;
+    }
     FusedArgs a{};
     a.x = x;
     a.rows = B;
