@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
+#include <iomanip>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -115,6 +116,7 @@ Args parse_args(int argc, char** argv) {
 void write_metadata(const Args& a, double wall, double rate, std::size_t n_failed) {
   std::ofstream out(a.out + ".meta.json");
   if (!out) die("cannot open metadata file: " + a.out + ".meta.json");
+  out.precision(17);  // masses and tolerances read back exactly
   const double amu = 1822.888486209;
   const std::size_t n = a.masses.size();
   out << "{\n  \"symbols\": [";

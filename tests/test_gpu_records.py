@@ -64,3 +64,31 @@ def test_generator_cli(gpu, tmp_path, graph):
     eng.sample(seed=11, n_sims=5000, radius=2.5, min_separation=1.0)
     eng.run(ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, h0_mode=ce.H0_FIXED, h0=1e-3))
     np.testing.assert_array_equal(eng.records(), rec)
+
+
+def test_verify_subset_against_fp64_asymptote(gpu, tmp_path):
+    """The verify_subset counterpart on a dataset the generator wrote with the
+    reference's production settings (pe_stop rule, redistribution)."""
+    exe = os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset")
+    out = str(tmp_path / "ch4.bin")
+    r = subprocess.run([exe, "--out", out, "--sims", "3000", "--batch", "1024", "--seed", "5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    from __graft_entry__ import load_package
+    ce = load_package()
+    import coulomb_explosion_amd.verify_subset as vs
+    res = vs.verify(ce, out, 500)
+    assert res["n"] == 500 and res["reference_status_ok"] == 1.0
+    # rtol 1e-4 / pe_stop 1e-5 with redistribution: well inside the 1% floor
+    assert res["over_floor"] == 0 and res["config_norm"]["max"] < 1e-2
+    assert res["config_norm"]["median"] < 1e-3
+
+
+def test_precision_sweep_small(gpu):
+    from __graft_entry__ import load_package
+    ce = load_package()
+    import coulomb_explosion_amd.precision_sweep as ps
+    for n in (2, 5, 8):
+        r = ps.sweep_one(ce, n, 1000, 7)
+        assert r["f64_failures"] == 0 and sum(r["f32_failures"].values()) == 0
+        assert r["dp_over_p"]["over_floor"] == 0 and r["dp_over_p"]["median"] < 1e-3
