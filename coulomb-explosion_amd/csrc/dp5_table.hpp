@@ -3,10 +3,10 @@
 // The tableau, error weights and controller constants are the method's
 // published values, as the reference prints them (src/integrators.cpp:203-226;
 // include/coulomb/batched_integrator.hpp:196-211); they have to be the same
-// numbers for the engine to be the same method.  From them this header derives,
-// in double at compile time, the tables of the Runge-Kutta-Nystrom arrangement
-// the kernel uses (dx/dt = v, so stage positions need only the acceleration
-// stages):
+// numbers for the engine to be the same method.  The tables of the
+// Runge-Kutta-Nystrom arrangement the kernel uses (dx/dt = v, so stage positions
+// need only the acceleration stages) follow from them and are written below as
+// exact rationals, each rounded once to double:
 //   c[s]     = sum_l a[s][l]                       (node of stage s)
 //   aa[s][l] = sum_{l<j<s} a[s][j] a[j][l]         (A*A)
 //   b[l]     = a[6][l]                             (5th-order weights)
