@@ -22,7 +22,7 @@ Other workloads (--workload):
          epochs) trained on 2M engine trajectories (untimed), then 8M fresh sims
          per step: plain DP5(4) to t_cut = 200, prior + ensemble residual where
          the members agree to 1e-5, integration continued elsewhere; timed
-         beside the plain method on the same sims, which also judges the error;
+         beside the plain method on the same sims, whose results set the error;
          a 50k subset is also compared with the engine's FP64 path.
   closure10  the same for N=10 (2 C, 2 O, 6 H; charges uniform {1,2,3}; R 3.5,
          min sep 0.9): 80 -> 512 x 4 -> 30, 4M training trajectories, 5M sims.

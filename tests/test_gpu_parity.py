@@ -14,7 +14,7 @@ from conftest import CH4_CHARGES, CH4_MASSES, per_atom_rel
 
 pytestmark = pytest.mark.gpu
 
-BASELINE_RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+HEADLINE_RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
 
 
 def ch4(ce):
@@ -72,10 +72,10 @@ def test_ch4_headline_rule_vs_oracle_lane_f32(gpu, oracle):
     momentum change < 1e-7 or t = 1e4, f32 state."""
     ce = gpu
     _, geom = oracle.sample(12345, 5, 2000, 2.5, 1.0)
-    eng, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **BASELINE_RULE)
+    eng, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **HEADLINE_RULE)
     from oracle import ffi
     ref = oracle.lane_run(CH4_MASSES, CH4_CHARGES, geom,
-                          ffi.make_config(rtol=1e-6, atol=1e-9, **BASELINE_RULE))
+                          ffi.make_config(rtol=1e-6, atol=1e-9, **HEADLINE_RULE))
     err = per_atom_rel(res.momenta, ref.momenta)
     assert err.max() < 1e-5, err.max()
     # v_rsq / v_rcp round differently from the oracle's 1/sqrt and 1/x, which
@@ -96,7 +96,7 @@ def test_ch4_f32_vs_reference_forms(gpu, oracle):
     ce = gpu
     from oracle import ffi
     _, geom = oracle.sample(49188291, 5, 512, 2.5, 1.0)
-    for rule in (BASELINE_RULE, dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7)):
+    for rule in (HEADLINE_RULE, dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7)):
         eng, res = engine_run(ce, ch4(ce), geom, 1e-6, 1e-9, **rule)
         cfg = ffi.make_config(rtol=1e-6, atol=1e-9, **rule)
         sc = oracle.scalar_run(CH4_MASSES, CH4_CHARGES, geom, cfg, precision="f32")
