@@ -1,0 +1,102 @@
+"""Edge cases through the C-ABI: empty batches, single atoms, closed forms on the
+GPU itself, infeasible sampling, determinism and the capacity boundary."""
+import numpy as np
+import pytest
+
+from conftest import CH4_MASSES
+
+pytestmark = pytest.mark.gpu
+
+RULE = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+
+
+def test_empty_batch(gpu):
+    ce = gpu
+    eng = ce.BatchedRK45(ce.Molecule([ce.Atom("H", 1837.15, 1.0)] * 3), 1.0, 16, 1e-6, 1e-9)
+    eng.sample(1, 0, 2.5, 1.0)
+    res = eng.run(ce.RunConfig(**RULE))
+    assert res.momenta.size == 0 and res.steps.size == 0 and res.batch_steps == 0
+    assert eng.stats()["sims"] == 0 and eng.stats()["attempted_steps"] == 0
+
+
+def test_single_atom(gpu):
+    """No pairs: zero force, zero momentum; the PE rule stops at once (PE0 = 0),
+    the momentum rule runs to t_max."""
+    ce = gpu
+    eng = ce.BatchedRK45(ce.Molecule([ce.Atom("C", 21874.66, 1.0)]), 1.0, 8, 1e-6, 1e-9)
+    eng.set_geometries(np.zeros((8, 1, 3)) + 0.5)
+    r = eng.run(ce.RunConfig(stop_rule=0, pe_stop_fraction=1e-5))
+    assert np.all(r.status == ce.SIM_OK_PE) and np.all(r.steps == 0)
+    np.testing.assert_array_equal(r.momenta, 0.0)
+    r = eng.run(ce.RunConfig(**RULE))
+    assert np.all(r.status == ce.SIM_OK_TMAX) and np.all(r.t_final == 1e4)
+    np.testing.assert_array_equal(r.momenta, 0.0)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 2e-6), (1, 1e-9)])
+def test_two_body_asymptote_closed_form(gpu, precision, tol):
+    """|p_inf| = sqrt(2 mu q1 q2 / r0), back to back along the axis (energy
+    conservation; the reference's two-proton case, test_reference.cpp:119-154)."""
+    ce = gpu
+    m1, m2, q1, q2 = 1837.15, 25526.04, 1.0, 2.0
+    mol = ce.Molecule([ce.Atom("A", m1, q1), ce.Atom("B", m2, q2)])
+    rng = np.random.default_rng(3)
+    n = 64
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    r0 = rng.uniform(0.8, 3.0, size=n)
+    geo = np.stack([axis * r0[:, None] * 0.5, -axis * r0[:, None] * 0.5], axis=1)
+    rtol, atol = (1e-6, 1e-9) if precision == 0 else (1e-10, 1e-14)
+    eng = ce.BatchedRK45(mol, 1.0, n, rtol, atol)
+    eng.set_geometries(geo)
+    res = eng.run(ce.RunConfig(stop_rule=0, pe_stop_fraction=1e-9, precision=precision))
+    p = res.momenta.reshape(n, 2, 3)
+    mu = m1 * m2 / (m1 + m2)
+    pinf = np.sqrt(2 * mu * q1 * q2 / r0)
+    np.testing.assert_allclose(np.linalg.norm(p[:, 0], axis=1), pinf, rtol=tol)
+    np.testing.assert_allclose(p[:, 0], axis * pinf[:, None], rtol=0, atol=tol * pinf.max() * 4)
+    np.testing.assert_allclose(p[:, 0] + p[:, 1], 0.0, atol=tol * pinf.max())
+
+
+def test_symmetric_ch4_closed_form(gpu):
+    """C at the centre of a regular H tetrahedron, unit charges: p_C = 0 and
+    |p_H| = sqrt(m_H E0 / 2) with E0 = 4 / r_CH + 6 / r_HH."""
+    ce = gpu
+    mC, mH = CH4_MASSES[0], CH4_MASSES[1]
+    mol = ce.Molecule([ce.Atom("C", mC, 1.0)] + [ce.Atom("H", mH, 1.0)] * 4)
+    tet = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / np.sqrt(3.0)
+    r_ch = 2.05
+    geo = np.concatenate([np.zeros((1, 3)), tet * r_ch])[None].repeat(4, axis=0)
+    e0 = 4 / r_ch + 6 / (r_ch * np.sqrt(8.0 / 3.0))
+    eng = ce.BatchedRK45(mol, 1.0, 4, 1e-10, 1e-14)
+    eng.set_geometries(geo)
+    res = eng.run(ce.RunConfig(stop_rule=0, pe_stop_fraction=1e-9, precision=1))
+    p = res.momenta.reshape(4, 5, 3)
+    assert np.abs(p[:, 0]).max() < 1e-9 * np.sqrt(mH * e0)
+    np.testing.assert_allclose(np.linalg.norm(p[:, 1:], axis=-1), np.sqrt(mH * e0 / 2), rtol=1e-9)
+
+
+def test_infeasible_sampling_is_flagged_not_fatal(gpu):
+    """Atoms that cannot be placed (min separation too large for the ball) mark
+    the sim SAMPLER_FAIL (the reference throws, sampler.cpp:50-54); the batch runs."""
+    ce = gpu
+    eng = ce.BatchedRK45(ce.Molecule([ce.Atom("H", 1837.15, 1.0)] * 5), 1.0, 32, 1e-6, 1e-9)
+    eng.sample(3, 32, 1.0, 1.9, max_attempts=200)
+    res = eng.run(ce.RunConfig(**RULE))
+    assert np.all(res.status == ce.SIM_SAMPLER_FAIL) and np.all(res.converged == 0)
+    assert np.all(res.steps == 0)
+
+
+def test_repeat_runs_are_bit_identical_and_fill_capacity(gpu):
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
+    n = 5000
+    eng = ce.BatchedRK45(mol, 1.0, n, 1e-6, 1e-9)
+    eng.sample(99, n, 2.5, 1.0)
+    a = eng.run(ce.RunConfig(**RULE))
+    eng.set_launch(block_threads=64, blocks_per_cu=1, refill_threshold=1, chunk=64)
+    b = eng.run(ce.RunConfig(**RULE))  # other launch shape: per-sim results do not change
+    np.testing.assert_array_equal(a.momenta, b.momenta)
+    np.testing.assert_array_equal(a.steps, b.steps)
+    with pytest.raises(ValueError):
+        eng.sample(99, n + 1, 2.5, 1.0)
