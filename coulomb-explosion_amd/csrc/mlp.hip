@@ -41,6 +41,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Round to nearest even with the gfx950 conversion instruction (v_cvt_pk_bf16_f32).
 __device__ __forceinline__ uint16_t f32_to_bf16_bits(float f) {
@@ -436,10 +437,8 @@ __device__ __forceinline__ void fused_hidden_epilogue(const float* bias,
       auto x1 = __builtin_amdgcn_permlane32_swap(P1, Q1, false, false);
       auto x2 = __builtin_amdgcn_permlane32_swap(R0, S0, false, false);
       auto x3 = __builtin_amdgcn_permlane32_swap(R1, S1, false, false);
-      const uint32_t lo[4] = {x0[0], x1[0], x0[1], x1[1]};
-      const uint32_t hi[4] = {x2[0], x3[0], x2[1], x3[1]};
-      act[2 * nb][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(lo));
-      act[2 * nb + 1][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(hi));
+      act[2 * nb][sb] = __builtin_bit_cast(bf16x8, u32x4{x0[0], x1[0], x0[1], x1[1]});
+      act[2 * nb + 1][sb] = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
     }
   }
 }
@@ -468,7 +467,7 @@ __global__ void __launch_bounds__(256) mlp_fused_forward(const FusedArgs a) {
         }
         d[j] = pack_bf16(f[0], f[1]);
       }
-      act[ks][sb] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(d));
+      act[ks][sb] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
     }
   const int total = K0C + (a.layers - 1) * FT::PER;
   int t = 0;
@@ -494,6 +493,159 @@ __global__ void __launch_bounds__(256) mlp_fused_forward(const FusedArgs a) {
     for (int r = 0; r < 16; ++r) {
       const int n = 8 * (r >> 2) + 4 * h + (r & 3);
       if (n < a.w_out && m < a.rows) a.y[m * a.w_out + n] = out[0][sb][r] + bias[n];
+    }
+  }
+}
+
+// H = 256 networks, output-block-major: the weights stream through LDS one
+// 32-row output block (all K columns) at a time, so a block's accumulators are
+// final after its tile and its SiLU epilogue (VALU) overlaps the next block's
+// MFMAs.  Each wave holds 32 samples: input fragments (act) and the next layer's
+// fragments (nxt) in registers, 64 VGPRs each, small enough for two waves per
+// SIMD, whose MFMA and VALU work also overlap.
+constexpr int kNoutTS = 256 + 8;  // LDS tile row stride (elements)
+
+// Tile t of the stream: layer 0 blocks 0..7 (K0 columns), hidden layers' blocks,
+// then the output layer's single block.
+__device__ __forceinline__ void nout_tile(const FusedArgs& a, int t, int k0, const uint16_t** W,
+                                          int* ldw, int* row0) {
+  if (t < 8) {
+    *W = a.W[0];
+    *ldw = k0;
+    *row0 = 32 * t;
+  } else {
+    const int l = 1 + (t - 8) / 8;
+    *W = a.W[l];
+    *ldw = 256;
+    *row0 = l == a.layers - 1 ? 0 : 32 * ((t - 8) % 8);
+  }
+}
+
+__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0, uint4 (&r)[4]) {
+  const uint16_t* W;
+  int ldw, row0;
+  nout_tile(a, t, k0, &W, &ldw, &row0);
+  const int per_row = ldw / 8;  // 16-byte chunks per row
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < 32 * per_row)
+      r[i] = *reinterpret_cast<const uint4*>(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
+  }
+}
+
+__device__ __forceinline__ void nout_stage(const FusedArgs& a, int t, int k0,
+                                           uint16_t (*T)[kNoutTS], const uint4 (&r)[4]) {
+  const int per_row = t < 8 ? k0 / 8 : 32;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < 32 * per_row) *reinterpret_cast<uint4*>(&T[e / per_row][8 * (e % per_row)]) = r[i];
+  }
+}
+
+// acc = W-tile (32 rows x 16 KS columns) x act[0..KS) for this wave's 32 samples.
+template <int KS>
+__device__ __forceinline__ void nout_mfma(const uint16_t (*T)[kNoutTS], const bf16x8 (&act)[16],
+                                          f32x16& acc) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const bf16x8 w = *reinterpret_cast<const bf16x8*>(&T[lane & 31][16 * ks + 8 * (lane >> 5)]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, act[ks], acc, 0, 0, 0);
+  }
+}
+
+// SiLU(acc + bias) of output block nb -> the next layer's fragments 2 nb, 2 nb + 1.
+__device__ __forceinline__ void nout_epilogue(const float* bias, int nb, const f32x16& acc,
+                                              bf16x8& lo_frag, bf16x8& hi_frag) {
+  const int h = (threadIdx.x & 63) >> 5;
+  float v[16];
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      v[4 * q + u] = silu(acc[4 * q + u] + bias[32 * nb + 8 * q + 4 * h + u]);
+  uint32_t P0 = pack_bf16(v[0], v[1]), P1 = pack_bf16(v[2], v[3]);
+  uint32_t Q0 = pack_bf16(v[4], v[5]), Q1 = pack_bf16(v[6], v[7]);
+  uint32_t R0 = pack_bf16(v[8], v[9]), R1 = pack_bf16(v[10], v[11]);
+  uint32_t S0 = pack_bf16(v[12], v[13]), S1 = pack_bf16(v[14], v[15]);
+  auto x0 = __builtin_amdgcn_permlane32_swap(P0, Q0, false, false);
+  auto x1 = __builtin_amdgcn_permlane32_swap(P1, Q1, false, false);
+  auto x2 = __builtin_amdgcn_permlane32_swap(R0, S0, false, false);
+  auto x3 = __builtin_amdgcn_permlane32_swap(R1, S1, false, false);
+  lo_frag = __builtin_bit_cast(bf16x8, u32x4{x0[0], x1[0], x0[1], x1[1]});
+  hi_frag = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
+}
+
+// One layer of 8 output blocks (KS k16-steps of input); t advances by 8.
+template <int KS>
+__device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const float* bias,
+                                           uint16_t (*tile)[32][kNoutTS], int& t, int total,
+                                           const bf16x8 (&act)[16], bf16x8 (&nxt)[16]) {
+  f32x16 prev;
+  static_for<0, 8>([&](auto NB) {
+    constexpr int nb = decltype(NB)::value;
+    __syncthreads();  // tile t staged; the other buffer is free
+    uint4 pre[4];
+    const bool more = t + 1 < total;
+    if (more) nout_fetch(a, t + 1, k0, pre);
+    f32x16 cur;
+    nout_mfma<KS>(tile[t & 1], act, cur);
+    if constexpr (nb > 0) nout_epilogue(bias, nb - 1, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
+    if (more) nout_stage(a, t + 1, k0, tile[(t + 1) & 1], pre);
+    prev = cur;
+    ++t;
+  });
+  nout_epilogue(bias, 7, prev, nxt[14], nxt[15]);
+}
+
+template <int K0S>
+__global__ void __launch_bounds__(256, 2) mlp_fused_forward_n256(const FusedArgs a) {
+  __shared__ __align__(16) uint16_t tile[2][32][kNoutTS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
+  const int64_t m = static_cast<int64_t>(blockIdx.x) * 128 + wave * 32 + ln;  // this lane's sample
+  constexpr int k0 = 16 * K0S;
+  bf16x8 act[16], nxt[16];
+#pragma unroll
+  for (int ks = 0; ks < 16; ++ks) {
+    uint32_t d[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      float f[2];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        const int k = 16 * ks + 8 * h + 2 * j + u;
+        f[u] = (ks < K0S && k < a.w0 && m < a.rows) ? a.x[m * a.w0 + k] : 0.0f;
+      }
+      d[j] = pack_bf16(f[0], f[1]);
+    }
+    act[ks] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
+  }
+  const int total = 8 * (a.layers - 1) + 1;
+  int t = 0;
+  {
+    uint4 pre[4];
+    nout_fetch(a, 0, k0, pre);
+    nout_stage(a, 0, k0, tile[0], pre);
+  }
+  nout_layer<K0S>(a, k0, a.b[0], tile, t, total, act, nxt);
+  for (int l = 1; l + 1 < a.layers; ++l) {
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) act[ks] = nxt[ks];
+    nout_layer<16>(a, k0, a.b[l], tile, t, total, act, nxt);
+  }
+  __syncthreads();  // output layer tile staged
+  f32x16 out;
+  nout_mfma<16>(tile[t & 1], nxt, out);
+  const float* bias = a.b[a.layers - 1];
+  if (m < a.rows) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int n = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (n < a.w_out) a.y[m * a.w_out + n] = out[r] + bias[n];
     }
   }
 }
@@ -1042,9 +1194,9 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
       a.b[l] = m->P + m->b_off[l];
     }
     a.y = y;
-    if (m->fused_h == 256) {  // 2 x 32 samples per wave, inputs in one 64-wide tile
-      const unsigned grid = static_cast<unsigned>((B + 255) / 256);
-      hipLaunchKernelGGL((mlp_fused_forward<256, 2, 1>), dim3(grid), dim3(256), 0, s, a);
+    if (m->fused_h == 256) {  // 32 samples per wave, output-block-major
+      const unsigned grid = static_cast<unsigned>((B + 127) / 128);
+      hipLaunchKernelGGL((mlp_fused_forward_n256<4>), dim3(grid), dim3(256), 0, s, a);
     } else {  // 32 samples per wave, inputs in four 32-wide tiles
       const unsigned grid = static_cast<unsigned>((B + 127) / 128);
       hipLaunchKernelGGL((mlp_fused_forward<512, 1, 4>), dim3(grid), dim3(256), 0, s, a);
