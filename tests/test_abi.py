@@ -119,3 +119,16 @@ def test_product_does_not_import_oracle():
             if fn.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, fn)).read()
                 assert "oracle" not in text.replace("oracle path", ""), fn
+
+
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    """include/coulomb_gpu.hpp builds a reference-style caller and links the library."""
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "mirror_demo.cpp")
+    exe = str(tmp_path / "mirror_demo")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+                        "-I" + os.path.join(ROOT, "include"), src, "-o", exe,
+                        "-L" + os.path.join(ROOT, "coulomb-explosion_amd"), "-lcoulomb_explosion",
+                        "-Wl,-rpath," + os.path.join(ROOT, "coulomb-explosion_amd")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
