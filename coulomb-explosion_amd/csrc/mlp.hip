@@ -321,8 +321,10 @@ struct FusedArgs {
   int w0;
   int layers;      // weight layers (>= 2): w0->H, (layers - 2) x H->H, H->w_out
   int w_out;
-  const uint16_t* W[kFusedMaxLayers];  // padded bf16 [round32(out)][round64(in)]
-  const float* b[kFusedMaxLayers];     // [out]
+  // Per-layer pointers in device memory (indexing a kernel-argument array with a
+  // run-time layer would copy the arguments to scratch):
+  const uint16_t* const* W;  // padded bf16 [round32(out)][round64(in)]
+  const float* const* b;     // [out]
   float* y;                            // [rows][w_out]
 };
 
@@ -521,26 +523,25 @@ __device__ __forceinline__ void nout_tile(const FusedArgs& a, int t, int k0, con
   }
 }
 
-__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0, uint4 (&r)[4]) {
+__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0, u32x4 (&r)[4]) {
   const uint16_t* W;
   int ldw, row0;
   nout_tile(a, t, k0, &W, &ldw, &row0);
   const int per_row = ldw / 8;  // 16-byte chunks per row
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    if (e < 32 * per_row)
-      r[i] = *reinterpret_cast<const uint4*>(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
+  for (int i = 0; i < 4; ++i) {  // every register written (chunks past the tile reload chunk 0)
+    const int e = threadIdx.x + 256 * i < 32 * per_row ? threadIdx.x + 256 * i : 0;
+    r[i] = *reinterpret_cast<const u32x4*>(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
   }
 }
 
 __device__ __forceinline__ void nout_stage(const FusedArgs& a, int t, int k0,
-                                           uint16_t (*T)[kNoutTS], const uint4 (&r)[4]) {
+                                           uint16_t (*T)[kNoutTS], const u32x4 (&r)[4]) {
   const int per_row = t < 8 ? k0 / 8 : 32;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int e = threadIdx.x + 256 * i;
-    if (e < 32 * per_row) *reinterpret_cast<uint4*>(&T[e / per_row][8 * (e % per_row)]) = r[i];
+    if (e < 32 * per_row) *reinterpret_cast<u32x4*>(&T[e / per_row][8 * (e % per_row)]) = r[i];
   }
 }
 
@@ -580,24 +581,32 @@ __device__ __forceinline__ void nout_epilogue(const float* bias, int nb, const f
   hi_frag = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
 }
 
-// One layer of 8 output blocks (KS k16-steps of input); t advances by 8.
+// One layer of 8 output blocks (KS k16-steps of input); t advances by 8 (a layer
+// starts at an even t, so the parity of t is that of the block).  Tile t + 1 is
+// in registers (r1 for odd t + 1, r0 for even) when block t starts: its global
+// loads were issued a whole block earlier; block t issues tile t + 2's.
 template <int KS>
 __device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const float* bias,
                                            uint16_t (*tile)[32][kNoutTS], int& t, int total,
-                                           const bf16x8 (&act)[16], bf16x8 (&nxt)[16]) {
+                                           const bf16x8 (&act)[16], bf16x8 (&nxt)[16],
+                                           u32x4 (&r0)[4], u32x4 (&r1)[4]) {
   f32x16 prev;
   static_for<0, 8>([&](auto NB) {
     constexpr int nb = decltype(NB)::value;
-    __syncthreads();  // tile t staged; the other buffer is free
-    uint4 pre[4];
-    const bool more = t + 1 < total;
-    if (more) nout_fetch(a, t + 1, k0, pre);
-    f32x16 cur;
-    nout_mfma<KS>(tile[t & 1], act, cur);
-    if constexpr (nb > 0) nout_epilogue(bias, nb - 1, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
-    if (more) nout_stage(a, t + 1, k0, tile[(t + 1) & 1], pre);
-    prev = cur;
-    ++t;
+    auto block = [&](u32x4 (&next)[4], u32x4 (&after)[4]) {  // tiles t + 1, t + 2
+      __syncthreads();  // tile t staged; the other buffer is free
+      if (t + 2 < total) nout_fetch(a, t + 2, k0, after);
+      f32x16 cur;
+      nout_mfma<KS>(tile[t & 1], act, cur);
+      if constexpr (nb > 0) nout_epilogue(bias, nb - 1, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
+      if (t + 1 < total) nout_stage(a, t + 1, k0, tile[(t + 1) & 1], next);
+      prev = cur;
+      ++t;
+    };
+    if constexpr (nb & 1)
+      block(r0, r1);
+    else
+      block(r1, r0);
   });
   nout_epilogue(bias, 7, prev, nxt[14], nxt[15]);
 }
@@ -626,16 +635,15 @@ __global__ void __launch_bounds__(256, 2) mlp_fused_forward_n256(const FusedArgs
   }
   const int total = 8 * (a.layers - 1) + 1;
   int t = 0;
-  {
-    uint4 pre[4];
-    nout_fetch(a, 0, k0, pre);
-    nout_stage(a, 0, k0, tile[0], pre);
-  }
-  nout_layer<K0S>(a, k0, a.b[0], tile, t, total, act, nxt);
+  u32x4 r0[4], r1[4];
+  nout_fetch(a, 0, k0, r0);
+  nout_stage(a, 0, k0, tile[0], r0);
+  nout_fetch(a, 1, k0, r1);  // tile 1 in flight
+  nout_layer<K0S>(a, k0, a.b[0], tile, t, total, act, nxt, r0, r1);
   for (int l = 1; l + 1 < a.layers; ++l) {
 #pragma unroll
     for (int ks = 0; ks < 16; ++ks) act[ks] = nxt[ks];
-    nout_layer<16>(a, k0, a.b[l], tile, t, total, act, nxt);
+    nout_layer<16>(a, k0, a.b[l], tile, t, total, act, nxt, r0, r1);
   }
   __syncthreads();  // output layer tile staged
   f32x16 out;
@@ -675,6 +683,7 @@ struct ce_mlp {
   bool use_fused = true;      // inference (ce_mlp_forward, closure) through it
   bool wp_stale = true;       // Wp older than the master parameters
   std::vector<uint16_t*> Wp;  // padded bf16 copies for the fused kernel
+  void** ptrs = nullptr;      // device table: Wp[0..L), then the bias pointers
   int64_t adam_t = 0;
   // batch workspace
   int64_t cap = 0;
@@ -722,6 +731,7 @@ void free_mlp(ce_mlp* m) {
   f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
   for (auto* p : m->Wb) f(p);
   for (auto* p : m->Wp) f(p);
+  f(m->ptrs);
   for (auto* p : m->act) f(p);
   for (auto* p : m->Z) f(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -952,6 +962,17 @@ int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp
     for (int l = 0; l < m->L && rc == CE_OK; ++l)
       rc = dalloc(&m->Wp[l], static_cast<int64_t>(round_up(widths[l + 1], 32)) *
                                  round_up(widths[l], 64));
+    if (rc == CE_OK) rc = dalloc(&m->ptrs, 2 * kFusedMaxLayers);
+    if (rc == CE_OK) {
+      std::vector<void*> table(2 * kFusedMaxLayers, nullptr);
+      for (int l = 0; l < m->L; ++l) {
+        table[l] = m->Wp[l];
+        table[kFusedMaxLayers + l] = m->P + m->b_off[l];
+      }
+      if (hipMemcpy(m->ptrs, table.data(), table.size() * sizeof(void*), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        rc = fail(CE_ERR_HIP, "pointer table upload");
+    }
   }
   if (rc == CE_OK) {
     std::vector<float> p;
@@ -1189,10 +1210,8 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     a.w0 = m->widths[0];
     a.layers = m->L;
     a.w_out = m->widths[m->L];
-    for (int l = 0; l < m->L; ++l) {
-      a.W[l] = m->Wp[l];
-      a.b[l] = m->P + m->b_off[l];
-    }
+    a.W = reinterpret_cast<const uint16_t* const*>(m->ptrs);
+    a.b = reinterpret_cast<const float* const*>(m->ptrs + kFusedMaxLayers);
     a.y = y;
     if (m->fused_h == 256) {  // 32 samples per wave, output-block-major
       const unsigned grid = static_cast<unsigned>((B + 127) / 128);
