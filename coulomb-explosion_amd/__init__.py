@@ -119,6 +119,7 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_engine_set_launch": ([P, P], C.c_int),
         "ce_engine_get_launch": ([P, P], C.c_int),
         "ce_engine_enable_trace": ([P, C.c_uint32], C.c_int),
+        "ce_engine_enable_energy_diagnostic": ([P, C.c_int32], C.c_int),
         "ce_engine_enable_state": ([P, C.c_int32], C.c_int),
         "ce_engine_get_state": ([P, P, P, P, P, S], C.c_int),
         "ce_engine_run_subset": ([P, P, P, S, P], C.c_int),
@@ -245,7 +246,7 @@ class Result:
     status: np.ndarray
     attempts: np.ndarray
     t_final: np.ndarray
-    energy_err: np.ndarray
+    energy_err: np.ndarray | None  # None unless enable_energy_diagnostic() was on for the run
     momenta_f32: np.ndarray
     trace_h: np.ndarray | None = None
 
@@ -313,6 +314,7 @@ class BatchedRK45:
         self._loaded = 0
         self._dirty = False
         self._trace_cap = 0
+        self._energy_diag = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -374,6 +376,12 @@ class BatchedRK45:
         _check(_lib.ce_engine_enable_trace(self._h, cap))
         self._trace_cap = int(cap)
 
+    def enable_energy_diagnostic(self, on: bool = True) -> None:
+        """Next runs compute (KE + PE - E0)/E0 per sim in FP64 (Result.energy_err;
+        None while off)."""
+        _check(_lib.ce_engine_enable_energy_diagnostic(self._h, 1 if on else 0))
+        self._energy_diag = bool(on)
+
     # -- run ----------------------------------------------------------------------
     def run(self, cfg: RunConfig | None = None, download: bool = True) -> Result | None:
         """batched_integrator.hpp:123-191."""
@@ -396,16 +404,15 @@ class BatchedRK45:
         att = np.zeros(ns, np.uint32)
         st = np.zeros(ns, np.uint8)
         tf = np.zeros(ns)
-        ee = np.zeros(ns)
+        ee = np.zeros(ns) if self._energy_diag else None
         tr = np.zeros((ns, self._trace_cap)) if self._trace_cap else None
 
         def ptr(a, ct):
-            return a.ctypes.data_as(C.POINTER(ct))
+            return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
 
         res = _Results(ptr(mom, C.c_double), ptr(mom32, C.c_float), ptr(steps, C.c_uint32),
                        ptr(att, C.c_uint32), ptr(st, C.c_uint8), ptr(tf, C.c_double),
-                       ptr(ee, C.c_double), ptr(tr, C.c_double) if tr is not None else None,
-                       self._trace_cap, 0)
+                       ptr(ee, C.c_double), ptr(tr, C.c_double), self._trace_cap, 0)
         _check(_lib.ce_engine_download(self._h, C.byref(res)))
         conv = ((st <= SIM_OK_TMAX) | (st == SIM_CLOSURE)).astype(np.uint8)
         return Result(mom.reshape(-1), steps, conv, int(steps.max()) if ns else 0, st, att, tf,
@@ -514,6 +521,7 @@ class MixedBatch:
         self._h = h
         self.total = sum(s.count for s in self.species)
         self._per_sim_charges = any(s.charge_levels > 0 for s in self.species)
+        self._energy_diag = False
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -597,6 +605,12 @@ class MixedBatch:
         _check(_lib.ce_engine_run(self._h, C.byref(c), None))
         return self.download() if download else None
 
+    def enable_energy_diagnostic(self, on: bool = True) -> None:
+        """Next runs compute (KE + PE - E0)/E0 per sim in FP64 ("energy_err";
+        None while off)."""
+        _check(_lib.ce_engine_enable_energy_diagnostic(self._h, 1 if on else 0))
+        self._energy_diag = bool(on)
+
     def download(self) -> list:
         """Per species: dict(momenta [count, n, 3], steps, attempts, status, t_final, energy_err)."""
         n = sum(s.count * 3 * s.molecule.size() for s in self.species)
@@ -605,19 +619,20 @@ class MixedBatch:
         att = np.zeros(self.total, np.uint32)
         st = np.zeros(self.total, np.uint8)
         tf = np.zeros(self.total)
-        ee = np.zeros(self.total)
+        ee = np.zeros(self.total) if self._energy_diag else None
 
         def ptr(a, ct):
-            return a.ctypes.data_as(C.POINTER(ct))
+            return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
 
         res = _Results(ptr(mom, C.c_double), None, ptr(steps, C.c_uint32), ptr(att, C.c_uint32),
                        ptr(st, C.c_uint8), ptr(tf, C.c_double), ptr(ee, C.c_double), None, 0, 0)
         _check(_lib.ce_engine_download(self._h, C.byref(res)))
         out = []
         for i, m in enumerate(self._split(mom, 3)):
-            sl = [self._split_sims(a)[i] for a in (steps, att, st, tf, ee)]
+            sl = [self._split_sims(a)[i] for a in (steps, att, st, tf)]
             out.append({"momenta": m, "steps": sl[0], "attempts": sl[1], "status": sl[2],
-                        "t_final": sl[3], "energy_err": sl[4]})
+                        "t_final": sl[3],
+                        "energy_err": self._split_sims(ee)[i] if ee is not None else None})
         return out
 
 

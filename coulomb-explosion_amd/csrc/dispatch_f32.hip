@@ -4,7 +4,7 @@
 
 namespace ce {
 hipError_t launch_dp5_f32(const LaunchReq& req, LaunchInfo* info, bool* bad_n) {
-  return req.per_sim_charges ? launch_all<float, true>(req, info, bad_n)
-                             : launch_all<float, false>(req, info, bad_n);
+  return req.per_sim_charges ? launch_all<float, true, true>(req, info, bad_n)
+                             : launch_all<float, false, true>(req, info, bad_n);
 }
 }  // namespace ce

@@ -201,7 +201,12 @@ int copy_results(ce_engine* e, ce_results* out) {
   if (rc == CE_OK && out->attempts) rc = copy_per_sim(e, out->attempts, e->attempts, sizeof(uint32_t), d2h);
   if (rc == CE_OK && out->status) rc = copy_per_sim(e, out->status, e->status, 1, d2h);
   if (rc == CE_OK && out->t_final) rc = copy_per_sim(e, out->t_final, e->tfin, sizeof(double), d2h);
-  if (rc == CE_OK && out->energy_err) rc = copy_per_sim(e, out->energy_err, e->eerr, sizeof(double), d2h);
+  if (rc == CE_OK && out->energy_err) {
+    if (!e->eerr_valid)
+      return fail(CE_ERR_INVALID_ARGUMENT,
+                  "energy_err: the last run did not compute it (ce_engine_enable_energy_diagnostic)");
+    rc = copy_per_sim(e, out->energy_err, e->eerr, sizeof(double), d2h);
+  }
   if (rc == CE_OK && out->trace_h != nullptr) {
     if (out->trace_cap != e->trace_cap || e->trace == nullptr)
       return fail(CE_ERR_INVALID_ARGUMENT, "trace_cap does not match the enabled trace");
@@ -521,6 +526,12 @@ int ce_engine_enable_trace(ce_engine* e, uint32_t cap) {
   return CE_OK;
 }
 
+int ce_engine_enable_energy_diagnostic(ce_engine* e, int32_t on) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  e->energy_diag = on != 0;
+  return CE_OK;
+}
+
 int ce_engine_set_launch(ce_engine* e, const ce_launch_config* lc) {
   if (e == nullptr || lc == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (lc->block_threads != 0 &&
@@ -657,7 +668,7 @@ int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, si
   a.attempts = e->attempts;
   a.status = e->status;
   a.t_final = e->tfin;
-  a.eerr = e->eerr;
+  a.eerr = e->energy_diag ? e->eerr : nullptr;
   a.trace_h = e->trace;
   a.trace_cap = e->trace_cap;
   a.stop_rule = cfg->stop_rule;
@@ -736,6 +747,7 @@ int finish_run(ce_engine* e, size_t n_run, const ce::LaunchInfo& info, int preci
   e->stats.n_cus = e->n_cus;
   e->stats.launches = info.launches;
   e->have_results = true;
+  e->eerr_valid = e->energy_diag;
   if (e->state_x != nullptr) e->state_prec = precision;
   return CE_OK;
 }
@@ -799,6 +811,7 @@ int ce_engine_capture(ce_engine* e, const ce_run_config* cfg, uint64_t seed, siz
   e->graph_info = info;
   e->graph_cfg = *cfg;
   e->graph_sims = n_sims;
+  e->graph_eerr = e->energy_diag;
   e->have_results = false;
   return CE_OK;
 }
@@ -817,6 +830,7 @@ int ce_engine_replay(ce_engine* e, uint64_t first_index, ce_results* out) {
   CE_HIP(hipEventRecord(e->k1, e->stream));
   const int rc = ce::finish_run(e, e->graph_sims, e->graph_info, e->graph_cfg.precision);
   if (rc != CE_OK) return rc;
+  e->eerr_valid = e->graph_eerr;
   e->stats.sampler_ms = 0.0;  // inside the graph, in kernel_ms
   return copy_results(e, out);
 }

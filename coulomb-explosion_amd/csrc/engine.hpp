@@ -57,6 +57,8 @@ struct ce_engine {
   double* eerr = nullptr;
   double* trace = nullptr;
   uint32_t trace_cap = 0;
+  bool energy_diag = false;  // compute eerr on the next runs
+  bool eerr_valid = false;   // the last run computed eerr
   uint8_t* records = nullptr;
   size_t records_cap = 0;
   // integration state at the stop (ce_engine_enable_state): positions and
@@ -73,6 +75,7 @@ struct ce_engine {
   uint64_t* first_dev = nullptr;  // first sim index of the replayed batch
   ce_run_config graph_cfg{};
   size_t graph_sims = 0;
+  bool graph_eerr = false;  // the captured integrator computes eerr
   ce::LaunchInfo graph_info{};
   // launch
   ce_launch_config launch = {0, 0, 4, 64};

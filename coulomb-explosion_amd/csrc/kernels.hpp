@@ -78,23 +78,42 @@ constexpr int pair_index(int i, int j) {  // i < j
   return i * N - i * (i + 1) / 2 + (j - i - 1);
 }
 
-// Pair constant k q_i q_j of pair (i, j): from the molecule table, or -- when
-// every simulation draws its own charges (PS) -- from the lane's charges.
-template <class T, int N, bool PS>
+// Where a launch takes its pair constants k q_i q_j from.
+enum : int {
+  kPairTable = 0,  // the molecule's table (kernel arguments)
+  kPerSim = 1,     // each simulation's own charges
+  // Every pair has the same constant c (equal charges, or N = 2): the forces are
+  // summed without c, which enters through the per-atom factors (im = c / m) and
+  // the potential energy (times c).  One multiply per pair less; for c = 1 the
+  // results are those of the table form bit for bit.
+  kUniform = 2,
+};
+
+// Pair constant k q_i q_j of pair (i, j) as the force sum uses it.
+template <class T, int N, int CM>
 struct PairK {
   const MolConst<T, N>& mc;
-  T kq[PS ? N : 1];  // k q_i
-  T q[PS ? N : 1];   // q_j
+  T kq[CM == kPerSim ? N : 1];  // k q_i
+  T q[CM == kPerSim ? N : 1];   // q_j
   template <int I, int J>
   __device__ __forceinline__ T get() const {
-    if constexpr (PS)
+    if constexpr (CM == kPerSim)
       return kq[I] * q[J];
+    else if constexpr (CM == kUniform)
+      return T(1);
     else
       return mc.pc[pair_index<N>(I, J)];
   }
+  // The potential energy summed with get() -> the potential energy.
+  __device__ __forceinline__ T pe(T u) const {
+    if constexpr (CM == kUniform)
+      return u * mc.pc[0];
+    else
+      return u;
+  }
   template <int I, int J>
   __device__ __forceinline__ double getd() const {
-    if constexpr (PS)
+    if constexpr (CM == kPerSim)
       return static_cast<double>(kq[I]) * static_cast<double>(q[J]);
     else
       return mc.pcd[pair_index<N>(I, J)];
@@ -115,9 +134,9 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 
 // Pair term: r = x_i - x_j, d2 = r.r (fma), inv = 1/|r|, s = k q_i q_j inv^3,
 // a_i += r s and a_j -= r s as single-rounding FMAs (Newton 3), then a_i *= 1/m_i.
-template <class T, int N, bool PS, bool WITH_PE, bool WITH_IM = true>
+template <class T, int N, int CM, bool WITH_PE, bool WITH_IM = true>
 __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T& pe,
-                                           const PairK<T, N, PS>& pk) {
+                                           const PairK<T, N, CM>& pk) {
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) a[q] = T(0);
   T u = T(0);
@@ -150,11 +169,11 @@ __device__ __forceinline__ void lane_force(const T (&x)[3 * N], T (&a)[3 * N], T
       a[3 * i + 2] *= pk.mc.im[i];
     }
   }
-  if constexpr (WITH_PE) pe = u;
+  if constexpr (WITH_PE) pe = pk.pe(u);
 }
 
-template <class T, int N, bool PS>
-__device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const PairK<T, N, PS>& pk) {
+template <class T, int N, int CM>
+__device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const PairK<T, N, CM>& pk) {
   T u = T(0);
   static_for<0, N>([&](auto I) {
     constexpr int i = decltype(I)::value;
@@ -167,7 +186,7 @@ __device__ __forceinline__ T lane_pe(const T (&x)[3 * N], const PairK<T, N, PS>&
       u = fmaT(pk.template get<i, j>(), rsqrtT(d2), u);
     });
   });
-  return u;
+  return pk.pe(u);
 }
 
 // Sum_{l < L} coef(l) * k[l][q] with zero coefficients skipped at compile time;
@@ -219,9 +238,9 @@ struct CoefEA {
 
 // Hairer / scipy initial step (src/integrators.cpp:197-250) for one lane, in
 // the scalar reference's accumulation order; k0 = a(x0).
-template <class T, int N, bool PS>
+template <class T, int N, int CM>
 __device__ T lane_hairer(const T (&x)[3 * N], const T (&v)[3 * N], const T (&a0)[3 * N], T rtol,
-                         T atol, T max_dt, const PairK<T, N, PS>& pk) {
+                         T atol, T max_dt, const PairK<T, N, CM>& pk) {
   T d0s = T(0), d1s = T(0);
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) {
@@ -244,7 +263,7 @@ __device__ T lane_hairer(const T (&x)[3 * N], const T (&v)[3 * N], const T (&a0)
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) xs[q] = x[q] + v[q] * h0;
   T unused;
-  lane_force<T, N, PS, false>(xs, a1, unused, pk);
+  lane_force<T, N, CM, false>(xs, a1, unused, pk);
   T d2s = T(0);
 #pragma unroll
   for (int q = 0; q < 3 * N; ++q) {
@@ -307,7 +326,7 @@ struct RunArgs {
   uint32_t* attempts;
   uint8_t* status;
   double* t_final;
-  double* eerr;
+  double* eerr;               // energy diagnostic per global sim, or nullptr (not computed)
   double* trace_h;            // [sim][trace_cap] or nullptr
   uint32_t trace_cap;
   int32_t stop_rule;
@@ -348,7 +367,7 @@ struct Totals {
 // one attempted step for every running lane, so the wave stays convergent
 // except for the short init / finalize branches.  Returns when the queue is
 // exhausted and every lane of the wave has finished.
-template <class T, int N, bool PS>
+template <class T, int N, int CM>
 __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
                                             const MolConst<T, N>& mc,
                                             unsigned long long* queue, Totals& tot) {
@@ -359,7 +378,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
   const T tol2 = static_cast<T>(p.momentum_tol * p.momentum_tol);
   const int lane = __lane_id();
 
-  PairK<T, N, PS> pk{mc};
+  PairK<T, N, CM> pk{mc};
   T x[N3], v[N3], k[7][N3];
   T h = T(0);
   double t = 0.0, e0 = 0.0, pe_stop = 0.0;
@@ -389,17 +408,6 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
       }
       const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0 / ke) : 1.0;
-      double pe_end = 0.0;
-      static_for<0, N>([&](auto I) {
-        constexpr int i = decltype(I)::value;
-        static_for<i + 1, N>([&](auto J) {
-          constexpr int j = decltype(J)::value;
-          const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
-          const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
-          const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
-          pe_end = fma(pk.template getd<i, j>(), rsqrt_f64(rx * rx + ry * ry + rz * rz), pe_end);
-        });
-      });
       const uint64_t base = sg.pos_off + loc * N3;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -428,7 +436,21 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       p.attempts[gs] = att;
       p.status[gs] = static_cast<uint8_t>(status);
       p.t_final[gs] = t;
-      p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+      if (p.eerr != nullptr) {
+        // energy diagnostic (KE + PE - E0) / E0 in FP64, when asked for
+        double pe_end = 0.0;
+        static_for<0, N>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          static_for<i + 1, N>([&](auto J) {
+            constexpr int j = decltype(J)::value;
+            const double rx = static_cast<double>(x[3 * i]) - static_cast<double>(x[3 * j]);
+            const double ry = static_cast<double>(x[3 * i + 1]) - static_cast<double>(x[3 * j + 1]);
+            const double rz = static_cast<double>(x[3 * i + 2]) - static_cast<double>(x[3 * j + 2]);
+            pe_end = fma(pk.template getd<i, j>(), rsqrt_f64(rx * rx + ry * ry + rz * rz), pe_end);
+          });
+        });
+        p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+      }
       if (p.trace_h != nullptr)
         for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
       active = false;
@@ -488,7 +510,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
             v[q] = T(0);
           }
         }
-        if constexpr (PS) {
+        if constexpr (CM == kPerSim) {
           const float* qs = p.qsim + sg.q_off + loc * N;
 #pragma unroll
           for (int i = 0; i < N; ++i) {
@@ -506,7 +528,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           e0 = p.state_e0[gs];
           pe_stop = p.pe_stop_fraction * e0;
           T unused;
-          lane_force<T, N, PS, false, false>(x, k[0], unused, pk);
+          lane_force<T, N, CM, false, false>(x, k[0], unused, pk);
           if (steps >= p.max_steps) status = kSimNotConverged;
         } else if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
           status = kSimSamplerFail;
@@ -514,7 +536,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         } else {
           T pe0;
           // The stages hold forces F = m a; 1/m enters via per-atom step factors.
-          lane_force<T, N, PS, true, false>(x, k[0], pe0, pk);
+          lane_force<T, N, CM, true, false>(x, k[0], pe0, pk);
           e0 = static_cast<double>(pe0);
           pe_stop = p.pe_stop_fraction * e0;
           if (p.stop_rule == kStopPE && e0 <= pe_stop) {
@@ -528,7 +550,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
               T a0[N3];
 #pragma unroll
               for (int q = 0; q < N3; ++q) a0[q] = k[0][q] * mc.im[q / 3];
-              h = lane_hairer<T, N, PS>(x, v, a0, rtol, atol, max_dt, pk);
+              h = lane_hairer<T, N, CM>(x, v, a0, rtol, atol, max_dt, pk);
             }
           }
         }
@@ -572,7 +594,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           }
         }
         T unused;
-        lane_force<T, N, PS, false, false>(xs, k[s], unused, pk);
+        lane_force<T, N, CM, false, false>(xs, k[s], unused, pk);
       });
       // Stage 7 (FSAL): the 5th-order solution; F(x_new) is the next k1.
 #pragma unroll
@@ -584,7 +606,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       }
       {
         T unused;
-        lane_force<T, N, PS, false, false>(xs, k[6], unused, pk);
+        lane_force<T, N, CM, false, false>(xs, k[6], unused, pk);
       }
       // Embedded error in the RMS norm, scale = atol + rtol max(|y_old|, |y_new|);
       // the squared norm is compared with 1.
@@ -606,14 +628,20 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
-#pragma unroll
-          for (int i = 0; i < N; ++i) {
+          // Every atom must pass; atom 0 is tested first and the others only when
+          // it passes on some lane of the wave (almost never before t_max).
+          const auto atom_settled = [&](int i) {
             const T dx = vn[3 * i] - v[3 * i], dy = vn[3 * i + 1] - v[3 * i + 1],
                     dz = vn[3 * i + 2] - v[3 * i + 2];
             const T num = fmaT(dz, dz, fmaT(dy, dy, dx * dx));
             const T den = fmaT(vn[3 * i + 2], vn[3 * i + 2],
                                fmaT(vn[3 * i + 1], vn[3 * i + 1], vn[3 * i] * vn[3 * i]));
-            settled = settled && (num < tol2 * den);
+            return num < tol2 * den;
+          };
+          settled = atom_settled(0);
+          if (__ballot(settled) != 0) {
+#pragma unroll
+            for (int i = 1; i < N; ++i) settled = settled && atom_settled(i);
           }
         }
 #pragma unroll
@@ -636,7 +664,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         ++tot.acc;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
-          if (static_cast<double>(lane_pe<T, N, PS>(x, pk)) <= pe_stop) status = kSimOkPE;
+          if (static_cast<double>(lane_pe<T, N, CM>(x, pk)) <= pe_stop) status = kSimOkPE;
         } else if (settled) {
           status = kSimOkMomentum;
         } else if (final_step) {
@@ -668,10 +696,10 @@ __device__ __forceinline__ void flush_totals(const RunArgs& p, Totals& tot) {
 }
 
 // One molecule for the whole launch (segment 0).
-template <class T, int N, bool PS>
+template <class T, int N, int CM>
 __global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
   Totals tot;
-  run_segment<T, N, PS>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
+  run_segment<T, N, CM>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
   flush_totals(p, tot);
 }
 
@@ -687,14 +715,14 @@ struct MixedConst {
   MolConst<T, N3> m3;
 };
 
-template <class T, int N0, int N1, int N2, int N3, bool PS>
+template <class T, int N0, int N1, int N2, int N3, int CM>
 __global__ void __launch_bounds__(256) dp5_mixed(const RunArgs p,
                                                  const MixedConst<T, N0, N1, N2, N3> mc) {
   Totals tot;
-  if (p.n_segments > 0) run_segment<T, N0, PS>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
-  if (p.n_segments > 1) run_segment<T, N1, PS>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);
-  if (p.n_segments > 2) run_segment<T, N2, PS>(p, p.seg[2], mc.m2, p.ctr + kCtrQueue + 2, tot);
-  if (p.n_segments > 3) run_segment<T, N3, PS>(p, p.seg[3], mc.m3, p.ctr + kCtrQueue + 3, tot);
+  if (p.n_segments > 0) run_segment<T, N0, CM>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
+  if (p.n_segments > 1) run_segment<T, N1, CM>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);
+  if (p.n_segments > 2) run_segment<T, N2, CM>(p, p.seg[2], mc.m2, p.ctr + kCtrQueue + 2, tot);
+  if (p.n_segments > 3) run_segment<T, N3, CM>(p, p.seg[3], mc.m3, p.ctr + kCtrQueue + 3, tot);
   flush_totals(p, tot);
 }
 
@@ -707,8 +735,12 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
       k0 += 0x9E3779B9u;
       k1 += 0xBB67AE85u;
     }
-    const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-    const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+    // 32 x 32 -> 64-bit products (one v_mad_u64_u32 each rather than a
+    // separate low and high multiply).
+    const uint64_t p0 = static_cast<uint64_t>(0xD2511F53u) * c[0];
+    const uint64_t p1 = static_cast<uint64_t>(0xCD9E8D57u) * c[2];
+    const uint32_t lo0 = static_cast<uint32_t>(p0), hi0 = static_cast<uint32_t>(p0 >> 32);
+    const uint32_t lo1 = static_cast<uint32_t>(p1), hi1 = static_cast<uint32_t>(p1 >> 32);
     const uint32_t n0 = hi1 ^ c[1] ^ k0;
     const uint32_t n2 = hi0 ^ c[3] ^ k1;
     c[0] = n0;

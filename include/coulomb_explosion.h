@@ -111,7 +111,8 @@ typedef struct {
   uint32_t* attempts;  /* attempted steps (accepted + rejected) */
   uint8_t* status;     /* ce_sim_status */
   double* t_final;     /* FP64 time at the stop */
-  double* energy_err;  /* (KE + PE - E0)/E0 at the stop, FP64, before redistribution */
+  double* energy_err;  /* (KE + PE - E0)/E0 at the stop, FP64, before redistribution;
+                          only after a run with ce_engine_enable_energy_diagnostic on */
   double* trace_h;     /* [sim][trace_cap] accepted step sizes (0 past the end) */
   uint32_t trace_cap;  /* 0 = no trace; must match ce_engine_enable_trace */
   uint32_t pad_;
@@ -225,6 +226,11 @@ int ce_engine_get_launch(const ce_engine* eng, ce_launch_config* lc);
 /* Records the first trace_cap accepted step sizes of every sim on the next
  * runs (the reference's StepObserver hook, driver.hpp:12-18); 0 disables. */
 int ce_engine_enable_trace(ce_engine* eng, uint32_t trace_cap);
+/* Energy diagnostic (no reference counterpart: BatchedRK45::Result has no energy
+ * error): with on != 0 the next runs compute (KE + PE - E0)/E0 per sim in FP64
+ * at the stop, for ce_results.energy_err.  Off by default: it adds an FP64
+ * potential-energy sum to every sim's finalize. */
+int ce_engine_enable_energy_diagnostic(ce_engine* eng, int32_t on);
 
 /* Saved integration state (no reference counterpart; the learned closure stops
  * sims at an early t_max and continues some of them).  When enabled, every run

@@ -119,7 +119,11 @@ void ceo_soa_accelerations_f32(size_t n, size_t k, const float* px, const float*
 void ceo_soa_potential_energy_f32(size_t n, size_t k, const float* px, const float* py,
                                   const float* pz, const float* pair_const, float* pe);
 
-/* ---- per-lane formulation executed by the GPU kernel ---- */
+/* ---- per-lane formulation executed by the GPU kernel ----
+ * ceo_lane_run_f32 takes the kernel's uniform form when every pair constant
+ * k q_i q_j is the same c (finite, non-zero): forces summed with constant 1,
+ * 1/m replaced by (float)(c / m), PE = c * sum (for c = 1 the same bits as the
+ * table form). */
 int ceo_lane_run_f32(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                      size_t n_sims, ceo_results* out);
 int ceo_lane_run_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,

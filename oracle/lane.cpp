@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "ce_oracle.h"
@@ -92,8 +93,9 @@ constexpr int kMaxN = 16;
 template <class T>
 struct LaneMol {
   int n;
-  T pc[kMaxN][kMaxN];  // k q_i q_j narrowed from double
-  T im[kMaxN];         // 1/m_i narrowed from double
+  T pc[kMaxN][kMaxN];  // k q_i q_j narrowed from double (1 in the uniform form)
+  T im[kMaxN];         // 1/m_i narrowed from double (c/m_i in the uniform form)
+  T pe_scale = T(1);   // the summed PE times this is the PE (c in the uniform form)
   double pcd[kMaxN][kMaxN];
   double md[kMaxN];
 };
@@ -112,8 +114,12 @@ void set_lane_charges(LaneMol<T>& r, const ceo_molecule* m, const double* q) {
   }
 }
 
+// The kernel's molecule constants.  With `uniform_ok` (the f32 kernel without
+// per-sim charges), a molecule whose pairs all have the same constant c (finite,
+// non-zero; always so for two atoms) takes the kernel's uniform form: forces
+// summed with pair constant 1, c folded into im = (T)(c / m_i), PE = c * sum.
 template <class T>
-LaneMol<T> lane_mol(const ceo_molecule* m) {
+LaneMol<T> lane_mol(const ceo_molecule* m, bool uniform_ok) {
   LaneMol<T> r{};
   r.n = m->n;
   for (int i = 0; i < m->n; ++i) {
@@ -122,6 +128,18 @@ LaneMol<T> lane_mol(const ceo_molecule* m) {
     for (int j = 0; j < m->n; ++j) {
       r.pcd[i][j] = m->coulomb_k * m->charges[i] * m->charges[j];
       r.pc[i][j] = static_cast<T>(r.pcd[i][j]);
+    }
+  }
+  bool uniform = uniform_ok && m->n >= 2;
+  const double c = uniform ? r.pcd[0][1] : 0.0;
+  uniform = uniform && c != 0.0 && std::isfinite(c);
+  for (int i = 0; uniform && i < m->n; ++i)
+    for (int j = i + 1; j < m->n; ++j) uniform = uniform && r.pcd[i][j] == c;
+  if (uniform) {
+    r.pe_scale = static_cast<T>(c);
+    for (int i = 0; i < m->n; ++i) {
+      r.im[i] = static_cast<T>(c / m->masses[i]);
+      for (int j = 0; j < m->n; ++j) r.pc[i][j] = T(1);
     }
   }
   return r;
@@ -156,7 +174,7 @@ void lane_force(const LaneMol<T>& m, const T* x, T* a, T* pe, bool apply_im = tr
       a[3 * i + 1] *= m.im[i];
       a[3 * i + 2] *= m.im[i];
     }
-  if (pe) *pe = u;
+  if (pe) *pe = u * m.pe_scale;
 }
 
 template <class T>
@@ -170,7 +188,7 @@ T lane_pe(const LaneMol<T>& m, const T* x) {
       const T d2 = std::fma(rx, rx, std::fma(ry, ry, rz * rz));
       u = std::fma(m.pc[i][j], T(1) / std::sqrt(d2), u);
     }
-  return u;
+  return u * m.pe_scale;
 }
 
 // Hairer / scipy initial step (src/integrators.cpp:197-250) for one lane, in the
@@ -408,7 +426,7 @@ template <class T>
 int lane_run(const ceo_molecule* cm, const ceo_config* cfg, const double* pos, size_t n_sims,
              ceo_results* out, const double* charges) {
   if (cm->n < 1 || cm->n > kMaxN) return -1;
-  LaneMol<T> m = lane_mol<T>(cm);
+  LaneMol<T> m = lane_mol<T>(cm, std::is_same<T, float>::value && charges == nullptr);
   for (size_t sim = 0; sim < n_sims; ++sim) {
     if (charges) set_lane_charges<T>(m, cm, charges + sim * cm->n);
     lane_run_one<T>(m, cfg, pos + sim * cm->n * 3, sim, out);

@@ -100,3 +100,27 @@ def test_repeat_runs_are_bit_identical_and_fill_capacity(gpu):
     np.testing.assert_array_equal(a.steps, b.steps)
     with pytest.raises(ValueError):
         eng.sample(99, n + 1, 2.5, 1.0)
+
+
+def test_energy_diagnostic_is_opt_in(gpu):
+    """Off by default (the reference's Result has no energy error): asking for it
+    is an error; switched on, the same integration also yields the FP64
+    diagnostic, and the momenta do not change."""
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
+    eng = ce.BatchedRK45(mol, 1.0, 2000, 1e-6, 1e-9)
+    eng.sample(5, 2000, 2.5, 1.0)
+    a = eng.run(ce.RunConfig(**RULE))
+    assert a.energy_err is None
+    ee = np.zeros(2000)
+    res = ce._Results(None, None, None, None, None, None,
+                      ee.ctypes.data_as(ce.C.POINTER(ce.C.c_double)), None, 0, 0)
+    with pytest.raises(ValueError):
+        ce._check(ce.library().ce_engine_download(eng._h, ce.C.byref(res)))
+    eng.enable_energy_diagnostic()
+    b = eng.run(ce.RunConfig(**RULE))
+    np.testing.assert_array_equal(a.momenta, b.momenta)
+    np.testing.assert_array_equal(a.steps, b.steps)
+    assert np.all(np.isfinite(b.energy_err)) and 0 < np.abs(b.energy_err).max() < 1e-4
+    eng.enable_energy_diagnostic(False)
+    assert eng.run(ce.RunConfig(**RULE)).energy_err is None

@@ -25,6 +25,7 @@ def ch4(ce):
 def engine_run(ce, mol, geom, rtol, atol, **cfg):
     n_sims = geom.shape[0]
     eng = ce.BatchedRK45(mol, 1.0, n_sims, rtol, atol)
+    eng.enable_energy_diagnostic()
     eng.set_geometries(geom)
     res = eng.run(ce.RunConfig(**cfg))
     return eng, res
@@ -88,6 +89,38 @@ def test_ch4_headline_rule_vs_oracle_lane_f32(gpu, oracle):
     st = eng.stats()
     assert st["accepted_steps"] == int(res.steps.sum())
     assert st["attempted_steps"] == int(res.attempts.sum())
+
+
+@pytest.mark.parametrize("masses,charges", [(CH4_MASSES, [3.0] * 5),
+                                             ([1837.15, 25526.04], [1.0, 3.0])])
+def test_uniform_pair_constant_vs_oracle_lane_f32(gpu, oracle, masses, charges):
+    """Molecules whose pairs share one constant c != 1 (all charges 3: c = 9;
+    two atoms: c = q1 q2 = 3) run the kernel with c folded into 1/m and the PE;
+    the oracle's f32 lane form does the same."""
+    ce = gpu
+    from oracle import ffi
+    n = len(masses)
+    _, geom = oracle.sample(4711, n, 1500, 2.5, 1.0)
+    mol = ce.Molecule([ce.Atom("X", m, q) for m, q in zip(masses, charges)])
+    for rule in (HEADLINE_RULE, dict(stop_rule=0, pe_stop_fraction=1e-6)):
+        _, res = engine_run(ce, mol, geom, 1e-6, 1e-9, **rule)
+        ref = oracle.lane_run(masses, charges, geom, ffi.make_config(rtol=1e-6, atol=1e-9, **rule))
+        assert per_atom_rel(res.momenta, ref.momenta).max() < 1e-5
+        dsteps = res.steps.astype(np.int64) - ref.steps.astype(np.int64)
+        assert np.mean(dsteps == 0) > 0.8
+        # Three times the charge explodes faster, and the momentum rule (|dp|/|p| <
+        # 1e-7 over a step) fires before t_max for some sims; at that threshold the
+        # v_rsq / 1/sqrt rounding difference can move the stop by a step or flip
+        # momentum <-> t_max for an odd sim.
+        diff = res.status != ref.status
+        assert np.mean(diff) < 0.01
+        assert set(np.unique(np.concatenate([res.status[diff], ref.status[diff]]))) <= {
+            ce.SIM_OK_MOMENTUM, ce.SIM_OK_TMAX}
+        same = ~diff
+        assert np.abs(dsteps[same]).max() <= 3
+        # the energy diagnostic (E0 carries the pair constant): f32-state level
+        assert np.abs(res.energy_err).max() < 1e-4
+        assert np.abs(res.energy_err - ref.energy_err)[same].max() < 1e-5
 
 
 def test_ch4_f32_vs_reference_forms(gpu, oracle):

@@ -237,6 +237,29 @@ def test_lane_form_equals_reference_form(oracle, rule):
     assert per_atom_rel(b.momenta, a.momenta).max() < 1e-5
 
 
+@pytest.mark.parametrize("rule", [dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1),
+                                  dict(stop_rule=0, pe_stop_fraction=1e-6)])
+def test_lane_uniform_form(oracle, rule):
+    """The f32 lane form folds a common pair constant c into 1/m and the PE.
+    For c = 1 it gives the table form's bits (per-sim-charge entry, pair
+    constant 1 * 1); for c = 9 (all charges 3) it agrees with the reference
+    form within 1e-5, and the PE scale enters E0 (pe-stop rule with
+    redistribution; energy diagnostic)."""
+    rc, geom = oracle.sample(77, 5, 48, 2.5, 1.0)
+    cfg = ffi.make_config(rtol=1e-6, atol=1e-9, **rule)
+    a = oracle.lane_run(CH4_MASSES, CH4_CHARGES, geom, cfg, precision="f32")
+    b = oracle.lane_run_q(CH4_MASSES, np.ones((48, 5)), geom, cfg, precision="f32")
+    np.testing.assert_array_equal(a.momenta, b.momenta)
+    np.testing.assert_array_equal(a.steps, b.steps)
+    q3 = [3.0] * 5
+    u = oracle.lane_run(CH4_MASSES, q3, geom, cfg, precision="f32")
+    ref = oracle.scalar_run(CH4_MASSES, q3, geom, cfg, precision="f32")
+    assert per_atom_rel(u.momenta, ref.momenta).max() < 1e-5
+    t = oracle.lane_run_q(CH4_MASSES, np.full((48, 5), 3.0), geom, cfg, precision="f32")
+    assert per_atom_rel(u.momenta, t.momenta).max() < 1e-5
+    assert np.abs(u.energy_err).max() < 1e-5
+
+
 # --------------------------------------------------------- lockstep form ----
 def test_batched_gate_vs_scalar(oracle):
     """bench/bench_batched_integrator.cpp:101-126: lockstep f64 vs per-sim scalar,
