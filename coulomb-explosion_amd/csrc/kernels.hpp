@@ -48,6 +48,17 @@ template <class T>
 __device__ __forceinline__ T minT(T a, T b) { return (b < a) ? b : a; }
 template <class T>
 __device__ __forceinline__ T maxT(T a, T b) { return (a < b) ? b : a; }
+// max(|a|, |b|) of the error scale.  The f32 path issues one v_max_f32 with abs
+// source modifiers: the compiler's fmaxf would first canonicalize both inputs
+// and std::max is a compare and a select.  Equal to std::max(|a|, |b|) unless an
+// input is NaN, and a NaN state gives a NaN error estimate (a rejected step)
+// either way.
+__device__ __forceinline__ float max_abs(float a, float b) {
+  float r;
+  asm("v_max_f32_e64 %0, |%1|, |%2|" : "=v"(r) : "v"(a), "v"(b));
+  return r;
+}
+__device__ __forceinline__ double max_abs(double a, double b) { return maxT(absT(a), absT(b)); }
 
 __device__ __forceinline__ uint64_t bcast64(uint64_t v) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(v));
@@ -615,8 +626,8 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       for (int q = 0; q < N3; ++q) {
         const T ex = stage_sum<T, 7>(k, q, CoefEA{});
         const T ev = stage_sum<T, 7>(k, q, CoefE{});
-        const T scx = fmaT(rtol, maxT(absT(x[q]), absT(xs[q])), atol);
-        const T scv = fmaT(rtol, maxT(absT(v[q]), absT(vn[q])), atol);
+        const T scx = fmaT(rtol, max_abs(x[q], xs[q]), atol);
+        const T scv = fmaT(rtol, max_abs(v[q], vn[q]), atol);
         const T rx = (g[q / 3] * ex) * rcpT(scx);
         const T rv = (gv[q / 3] * ev) * rcpT(scv);
         exs = fmaT(rx, rx, exs);
