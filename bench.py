@@ -184,6 +184,21 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
                           momentum_tol=MOM_TOL, h0_mode=ffi.H0_FIXED, h0=H0)
     r = o.cpu_baseline(CH4_MASSES, CH4_CHARGES, geom, cfg, threads, reps)
     one = o.cpu_baseline(CH4_MASSES, CH4_CHARGES, geom[: n_sims // 8], cfg, 1, 1)
+    # Fidelity of the port: one core at the reference's own benchmark setup
+    # (C 12.011 / H 1.008 amu, R 4.0, min sep 0.25, rtol 1e-4, atol 1e-7, PE stop
+    # 1e-5, Hairer start), where the reference's Highway engine is published at
+    # about 255k sims/s with 8 f32 lanes.
+    amu = 1822.888486209
+    rc, g2 = o.sample(SEED, 5, 40000, 4.0, 0.25)
+    cfg2 = ffi.make_config(rtol=1e-4, atol=1e-7, stop_rule=ffi.STOP_PE_FRACTION,
+                           pe_stop_fraction=1e-5)
+    fid = o.cpu_baseline([12.011 * amu] + [1.008 * amu] * 4, CH4_CHARGES, g2, cfg2, 1, 3)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {
         "value": r["sims"] / r["median_s"],
         "unit": "sims/s",
@@ -194,6 +209,9 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
                    f"instance per thread, median of {reps} passes"),
         "single_core_sims_per_s": one["sims"] / one["median_s"],
         "mean_batch_steps": r["batch_steps"] / max(1, r["sims"] // r["lanes"]),
+        "cpu_model": model,
+        "host_threads": os.cpu_count(),
+        "single_core_at_reference_setup_sims_per_s": fid["sims"] / fid["median_s"],
     }
 
 
