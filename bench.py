@@ -193,6 +193,11 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
     cfg2 = ffi.make_config(rtol=1e-4, atol=1e-7, stop_rule=ffi.STOP_PE_FRACTION,
                            pe_stop_fraction=1e-5)
     fid = o.cpu_baseline([12.011 * amu] + [1.008 * amu] * 4, CH4_CHARGES, g2, cfg2, 1, 3)
+    # Lockstep efficiency of the baseline on this workload, as the reference
+    # defines it (bench/bench_explosion.cpp:298-336): scalar steps of the sims /
+    # (K x shared-dt iterations); scalar steps from the per-lane form on a subset.
+    sub = o.lane_run(CH4_MASSES, CH4_CHARGES, geom[:4096], cfg)
+    mean_batch = r["batch_steps"] / max(1, r["sims"] // r["lanes"])
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -208,7 +213,8 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
                    f"{SEED}), lockstep BatchedRK45<float> restatement at {r['isa']}, one "
                    f"instance per thread, median of {reps} passes"),
         "single_core_sims_per_s": one["sims"] / one["median_s"],
-        "mean_batch_steps": r["batch_steps"] / max(1, r["sims"] // r["lanes"]),
+        "mean_batch_steps": mean_batch,
+        "lockstep_efficiency": float(sub.steps.mean()) / mean_batch,
         "cpu_model": model,
         "host_threads": os.cpu_count(),
         "single_core_at_reference_setup_sims_per_s": fid["sims"] / fid["median_s"],
@@ -456,7 +462,7 @@ def main() -> int:
     for _ in range(args.warmup):
         w.sample()
         w.run(cfg)
-    kernel_ms, sampler_ms, attempts, accepted, launches = [], [], 0, 0, 0
+    kernel_ms, sampler_ms, attempts, accepted, launches, slots = [], [], 0, 0, 0, 0
     print(f"[bench] timing {args.steps} steps", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -467,6 +473,7 @@ def main() -> int:
         kernel_ms.append(st["kernel_ms"])
         attempts += st["attempted_steps"]
         accepted += st["accepted_steps"]
+        slots += st["lane_slots"]
         launches = st["launches"]
     elapsed = time.perf_counter() - t0
     per_size = w.momenta_status()
@@ -517,6 +524,10 @@ def main() -> int:
             "converged_per_step": n_ok,
             "mean_accepted_steps": accepted / (args.sims * args.steps),
             "mean_attempted_steps": attempts / (args.sims * args.steps),
+            # attempted lane-steps / (64 x wave loop iterations): the per-lane-dt
+            # counterpart of the reference's shared-dt lockstep efficiency
+            # (bench/bench_explosion.cpp:298-336), ceiling 1
+            "lane_utilization": attempts / slots if slots else None,
             "kernel_ms_avg": avg_kernel_s * 1e3,
             "sampler_ms_avg": float(np.mean(sampler_ms)),
             "launches_per_step": launches,

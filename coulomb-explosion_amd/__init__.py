@@ -73,7 +73,7 @@ class _RunStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("sampler_ms", C.c_double), ("pack_ms", C.c_double),
                 ("sims", C.c_uint64),
                 ("attempted_steps", C.c_uint64), ("accepted_steps", C.c_uint64),
-                ("refills", C.c_uint64), ("grid_blocks", C.c_int32),
+                ("refills", C.c_uint64), ("lane_slots", C.c_uint64), ("grid_blocks", C.c_int32),
                 ("block_threads", C.c_int32), ("n_cus", C.c_int32), ("launches", C.c_int32)]
 
 

@@ -742,6 +742,7 @@ int finish_run(ce_engine* e, size_t n_run, const ce::LaunchInfo& info, int preci
   e->stats.attempted_steps = host_ctr[0];
   e->stats.accepted_steps = host_ctr[1];
   e->stats.refills = host_ctr[2];
+  e->stats.lane_slots = host_ctr[3];
   e->stats.grid_blocks = info.grid;
   e->stats.block_threads = info.block;
   e->stats.n_cus = e->n_cus;

@@ -322,7 +322,8 @@ struct Segment {
 };
 constexpr int kMaxSegments = 4;
 // ctr layout: [0] attempted steps, [1] accepted steps, [2] refill events,
-// [4 + s] work-queue head of segment s.
+// [3] lane slots (64 x loop iterations of every wave), [4 + s] work-queue head
+// of segment s.
 constexpr int kCtrQueue = 4;
 constexpr int kCtrWords = 8;
 
@@ -368,6 +369,7 @@ struct RunArgs {
 
 struct Totals {
   uint32_t att = 0, acc = 0, refills = 0;
+  uint32_t iters = 0;  // loop iterations of the wave (the same on every lane)
 };
 
 // Runs one segment to exhaustion with a persistent per-lane DP5(4): each lane
@@ -568,6 +570,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       }
     }
     if (__ballot(active) == 0) break;
+    ++tot.iters;
 
     // ---- one attempted DP5(4) step on every running lane ----
     if (active && status == kRunning) {
@@ -703,6 +706,7 @@ __device__ __forceinline__ void flush_totals(const RunArgs& p, Totals& tot) {
     atomicAdd(p.ctr + 0, static_cast<unsigned long long>(tot.att));
     atomicAdd(p.ctr + 1, static_cast<unsigned long long>(tot.acc));
     atomicAdd(p.ctr + 2, static_cast<unsigned long long>(tot.refills));
+    atomicAdd(p.ctr + 3, 64ull * tot.iters);
   }
 }
 

@@ -124,3 +124,18 @@ def test_energy_diagnostic_is_opt_in(gpu):
     assert np.all(np.isfinite(b.energy_err)) and 0 < np.abs(b.energy_err).max() < 1e-4
     eng.enable_energy_diagnostic(False)
     assert eng.run(ce.RunConfig(**RULE)).energy_err is None
+
+
+def test_lane_slot_accounting(gpu):
+    """stats()["lane_slots"] is 64 x the waves' loop iterations: a multiple of 64,
+    at least the attempted steps (one attempt per running lane per iteration),
+    and for a large batch mostly used (refill keeps the lanes busy)."""
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
+    eng = ce.BatchedRK45(mol, 1.0, 200000, 1e-6, 1e-9)
+    eng.sample(8, 200000, 2.5, 1.0)
+    eng.run(ce.RunConfig(**RULE), download=False)
+    st = eng.stats()
+    assert st["lane_slots"] % 64 == 0
+    assert st["attempted_steps"] <= st["lane_slots"]
+    assert st["attempted_steps"] > 0.5 * st["lane_slots"]
