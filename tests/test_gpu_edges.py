@@ -132,8 +132,8 @@ def test_lane_slot_accounting(gpu):
     and for a large batch mostly used (refill keeps the lanes busy)."""
     ce = gpu
     mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
-    eng = ce.BatchedRK45(mol, 1.0, 200000, 1e-6, 1e-9)
-    eng.sample(8, 200000, 2.5, 1.0)
+    eng = ce.BatchedRK45(mol, 1.0, 1000000, 1e-6, 1e-9)
+    eng.sample(8, 1000000, 2.5, 1.0)
     eng.run(ce.RunConfig(**RULE), download=False)
     st = eng.stats()
     assert st["lane_slots"] % 64 == 0
