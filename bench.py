@@ -364,8 +364,8 @@ def run_closure(ce, args) -> dict:
                          "final_loss": [float(h[-1]) for h in ens.losses]},
         },
         "roofline": {
-            "bound": "mfma", "kernel": "fused ensemble inference (mlp_fused_forward_n256 for 256-wide, "
-                                       "K-outer fused kernel for 512-wide), with features and gate",
+            "bound": "mfma", "kernel": "fused ensemble inference (mlp_fused_forward_nout), "
+                                       "with features and gate",
             "achieved": infer_tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": infer_tf / 2500.0,
             "traffic": None, "flops_per_sim_per_member": mlp_flops},
     }

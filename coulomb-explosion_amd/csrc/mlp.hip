@@ -305,10 +305,10 @@ __global__ void adam(float* p, const float* g, float* m, float* v, int64_t n, fl
 // H = 512 with w0 <= 128; w_out <= 32) in one kernel, computed transposed:
 // C^T[n][m] = sum_k W[n][k] act[m][k], so the weights are the MFMA A operand,
 // read from LDS, and the activations are the B operand, held in registers for
-// the whole network.  A workgroup owns 4 waves x 32 SB samples; the weights
-// stream through LDS in tiles of H rows x KC columns (32 KB), double-buffered
-// and shared by the 4 waves, the next tile's global loads in flight while the
-// current one feeds the MFMAs.  Between layers each lane turns its accumulators
+// the whole network.  A workgroup owns 4 waves x 32 samples; the weights stream
+// through LDS in 32-row output blocks (all K columns), double-buffered and shared
+// by the 4 waves, later tiles' global loads in flight while the current one feeds
+// the MFMAs.  Between layers each lane turns its accumulators
 // (features 8q + 4h + t of a 32-feature block, h = lane / 32) into the next
 // layer's B fragments (8 consecutive features per lane) with one
 // v_permlane32_swap per dword pair.  Products and their K order are those of
@@ -328,227 +328,67 @@ struct FusedArgs {
   float* y;                            // [rows][w_out]
 };
 
+// Output-block-major fused inference (H = 256 or 512): the weights stream through
+// LDS one 32-row output block (all K columns) at a time, so a block's accumulators
+// are final after its tile and its SiLU epilogue (VALU) overlaps the next block's
+// MFMAs.  Each wave holds 32 samples: the input fragments (act) and the next
+// layer's fragments (nxt) in registers, H / 4 VGPRs each -- two waves per SIMD at
+// H = 256 (their MFMA and VALU work also overlap), one at H = 512.
 template <int H>
-struct FusedTiles {
-  static constexpr int KC = 16384 / H;  // k columns per tile (32 KB of bf16)
-  static constexpr int TSTR = KC + 8;   // LDS row stride in elements
-  static constexpr int PER = H / KC;    // tiles of one H-deep layer
+struct Nout {
+  static constexpr int TS = H + 8;         // LDS tile row stride (elements)
+  static constexpr int NB = H / 32;        // output blocks of a hidden layer
+  static constexpr int KS = H / 16;        // k16 steps of a hidden layer's input
+  static constexpr int LD = 32 * H / 2048; // 16-byte chunks per thread of a full tile
 };
 
-// Loads tile t of the layer stream into registers: 8 x 16 bytes per thread.
-template <int H, int K0C>
-__device__ __forceinline__ void fused_fetch(const FusedArgs& a, int t, uint4 (&r)[8]) {
-  using FT = FusedTiles<H>;
-  int layer, chunk;
-  if (t < K0C) {
-    layer = 0;
-    chunk = t;
-  } else {
-    layer = 1 + (t - K0C) / FT::PER;
-    chunk = (t - K0C) % FT::PER;
-  }
-  const int rows = layer == a.layers - 1 ? 32 : H;
-  const int ldw = layer == 0 ? K0C * FT::KC : H;
-  const uint16_t* W = a.W[layer] + chunk * FT::KC;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int e = threadIdx.x + 256 * i;  // 8-element chunk id
-    const int row = e / (FT::KC / 8), c8 = e % (FT::KC / 8);
-    if (row < rows) r[i] = *reinterpret_cast<const uint4*>(W + row * ldw + 8 * c8);
-  }
-}
-
+// Tile t of the stream: layer 0's NB blocks (k0 columns), the hidden layers'
+// blocks, then the output layer's single block.
 template <int H>
-__device__ __forceinline__ void fused_stage(uint16_t (*T)[FusedTiles<H>::TSTR], int rows,
-                                            const uint4 (&r)[8]) {
-  using FT = FusedTiles<H>;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    const int row = e / (FT::KC / 8), c8 = e % (FT::KC / 8);
-    if (row < rows) *reinterpret_cast<uint4*>(&T[row][8 * c8]) = r[i];
-  }
-}
-
-// One layer over the tile stream: acc[nb][sb] += W-tile x act for the layer's
-// PER_L tiles (t advances).  NB = output blocks of 32 (H / 32, or 1 for the
-// output layer); KSL = k16 steps of this layer's input.
-template <int H, int K0C, int SB, int NB, int PER_L>
-__device__ __forceinline__ void fused_layer_t(const FusedArgs& a,
-                                              uint16_t (*tile)[H][FusedTiles<H>::TSTR], int& t,
-                                              int total, const bf16x8 (&act)[H / 16][SB],
-                                              f32x16 (&acc)[NB][SB]) {
-  using FT = FusedTiles<H>;
-  const int lane = threadIdx.x & 63, ln = lane & 31, h = lane >> 5;
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-#pragma unroll
-    for (int j = 0; j < SB; ++j)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
-  static_for<0, PER_L>([&](auto C) {
-    constexpr int c = decltype(C)::value;
-    __syncthreads();  // tile t staged by every thread; the other buffer is free
-    uint4 pre[8];
-    const bool more = t + 1 < total;
-    if (more) fused_fetch<H, K0C>(a, t + 1, pre);
-    const uint16_t (*T)[FT::TSTR] = tile[t & 1];
-    static_for<0, FT::KC / 16>([&](auto KK) {
-      constexpr int kk = decltype(KK)::value;
-      constexpr int ks = c * (FT::KC / 16) + kk;
-#pragma unroll
-      for (int nb = 0; nb < NB; ++nb) {
-        const bf16x8 w = *reinterpret_cast<const bf16x8*>(&T[32 * nb + ln][16 * kk + 8 * h]);
-#pragma unroll
-        for (int sb = 0; sb < SB; ++sb)
-          acc[nb][sb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, act[ks][sb], acc[nb][sb], 0, 0, 0);
-      }
-    });
-    if (more) {
-      int layer = t + 1 < K0C ? 0 : 1 + (t + 1 - K0C) / FT::PER;
-      fused_stage<H>(tile[(t + 1) & 1], layer == a.layers - 1 ? 32 : H, pre);
-    }
-    ++t;
-  });
-}
-
-template <int H, int SB>
-__device__ __forceinline__ void fused_hidden_epilogue(const float* bias,
-                                                      const f32x16 (&acc)[H / 32][SB],
-                                                      bf16x8 (&act)[H / 16][SB]) {
-  const int h = (threadIdx.x & 63) >> 5;
-#pragma unroll
-  for (int nb = 0; nb < H / 32; ++nb) {
-    float bv[16];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int u = 0; u < 4; ++u) bv[4 * q + u] = bias[32 * nb + 8 * q + 4 * h + u];
-#pragma unroll
-    for (int sb = 0; sb < SB; ++sb) {
-      float v[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) v[r] = silu(acc[nb][sb][r] + bv[r]);
-      // lane holds features 8q + 4h + t (q = r / 4, t = r % 4) of this block
-      uint32_t P0 = pack_bf16(v[0], v[1]), P1 = pack_bf16(v[2], v[3]);
-      uint32_t Q0 = pack_bf16(v[4], v[5]), Q1 = pack_bf16(v[6], v[7]);
-      uint32_t R0 = pack_bf16(v[8], v[9]), R1 = pack_bf16(v[10], v[11]);
-      uint32_t S0 = pack_bf16(v[12], v[13]), S1 = pack_bf16(v[14], v[15]);
-      // lanes 0-31 take lanes 32-63's (P, R); lanes 32-63 take lanes 0-31's (Q, S)
-      auto x0 = __builtin_amdgcn_permlane32_swap(P0, Q0, false, false);
-      auto x1 = __builtin_amdgcn_permlane32_swap(P1, Q1, false, false);
-      auto x2 = __builtin_amdgcn_permlane32_swap(R0, S0, false, false);
-      auto x3 = __builtin_amdgcn_permlane32_swap(R1, S1, false, false);
-      act[2 * nb][sb] = __builtin_bit_cast(bf16x8, u32x4{x0[0], x1[0], x0[1], x1[1]});
-      act[2 * nb + 1][sb] = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
-    }
-  }
-}
-
-template <int H, int SB, int K0C>
-__global__ void __launch_bounds__(256) mlp_fused_forward(const FusedArgs a) {
-  using FT = FusedTiles<H>;
-  __shared__ __align__(16) uint16_t tile[2][H][FT::TSTR];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
-  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (4 * 32 * SB) + wave * 32 * SB;
-  // inputs: lane holds sample m0 + 32 sb + ln, features 16 ks + 8 h + 0..7
-  bf16x8 act[H / 16][SB];
-#pragma unroll
-  for (int ks = 0; ks < H / 16; ++ks)
-#pragma unroll
-    for (int sb = 0; sb < SB; ++sb) {
-      uint32_t d[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        float f[2];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int k = 16 * ks + 8 * h + 2 * j + u;
-          const int64_t m = m0 + 32 * sb + ln;
-          f[u] = (ks < K0C * FT::KC / 16 && k < a.w0 && m < a.rows) ? a.x[m * a.w0 + k] : 0.0f;
-        }
-        d[j] = pack_bf16(f[0], f[1]);
-      }
-      act[ks][sb] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
-    }
-  const int total = K0C + (a.layers - 1) * FT::PER;
-  int t = 0;
-  {
-    uint4 pre[8];
-    fused_fetch<H, K0C>(a, 0, pre);
-    fused_stage<H>(tile[0], H, pre);
-  }
-  f32x16 acc[H / 32][SB];
-  fused_layer_t<H, K0C, SB, H / 32, K0C>(a, tile, t, total, act, acc);
-  fused_hidden_epilogue<H, SB>(a.b[0], acc, act);
-  for (int l = 1; l + 1 < a.layers; ++l) {
-    fused_layer_t<H, K0C, SB, H / 32, FT::PER>(a, tile, t, total, act, acc);
-    fused_hidden_epilogue<H, SB>(a.b[l], acc, act);
-  }
-  f32x16 out[1][SB];
-  fused_layer_t<H, K0C, SB, 1, FT::PER>(a, tile, t, total, act, out);
-  const float* bias = a.b[a.layers - 1];
-#pragma unroll
-  for (int sb = 0; sb < SB; ++sb) {
-    const int64_t m = m0 + 32 * sb + ln;
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int n = 8 * (r >> 2) + 4 * h + (r & 3);
-      if (n < a.w_out && m < a.rows) a.y[m * a.w_out + n] = out[0][sb][r] + bias[n];
-    }
-  }
-}
-
-// H = 256 networks, output-block-major: the weights stream through LDS one
-// 32-row output block (all K columns) at a time, so a block's accumulators are
-// final after its tile and its SiLU epilogue (VALU) overlaps the next block's
-// MFMAs.  Each wave holds 32 samples: input fragments (act) and the next layer's
-// fragments (nxt) in registers, 64 VGPRs each, small enough for two waves per
-// SIMD, whose MFMA and VALU work also overlap.
-constexpr int kNoutTS = 256 + 8;  // LDS tile row stride (elements)
-
-// Tile t of the stream: layer 0 blocks 0..7 (K0 columns), hidden layers' blocks,
-// then the output layer's single block.
 __device__ __forceinline__ void nout_tile(const FusedArgs& a, int t, int k0, const uint16_t** W,
                                           int* ldw, int* row0) {
-  if (t < 8) {
+  constexpr int NB = Nout<H>::NB;
+  if (t < NB) {
     *W = a.W[0];
     *ldw = k0;
     *row0 = 32 * t;
   } else {
-    const int l = 1 + (t - 8) / 8;
+    const int l = 1 + (t - NB) / NB;
     *W = a.W[l];
-    *ldw = 256;
-    *row0 = l == a.layers - 1 ? 0 : 32 * ((t - 8) % 8);
+    *ldw = H;
+    *row0 = l == a.layers - 1 ? 0 : 32 * ((t - NB) % NB);
   }
 }
 
-__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0, u32x4 (&r)[4]) {
+template <int H>
+__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0,
+                                           u32x4 (&r)[Nout<H>::LD]) {
   const uint16_t* W;
   int ldw, row0;
-  nout_tile(a, t, k0, &W, &ldw, &row0);
+  nout_tile<H>(a, t, k0, &W, &ldw, &row0);
   const int per_row = ldw / 8;  // 16-byte chunks per row
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {  // every register written (chunks past the tile reload chunk 0)
+  for (int i = 0; i < Nout<H>::LD; ++i) {  // every register written (chunks past the tile reload chunk 0)
     const int e = threadIdx.x + 256 * i < 32 * per_row ? threadIdx.x + 256 * i : 0;
     r[i] = *reinterpret_cast<const u32x4*>(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
   }
 }
 
-__device__ __forceinline__ void nout_stage(const FusedArgs& a, int t, int k0,
-                                           uint16_t (*T)[kNoutTS], const u32x4 (&r)[4]) {
-  const int per_row = t < 8 ? k0 / 8 : 32;
+template <int H>
+__device__ __forceinline__ void nout_stage(int t, int k0, uint16_t (*T)[Nout<H>::TS],
+                                           const u32x4 (&r)[Nout<H>::LD]) {
+  const int per_row = t < Nout<H>::NB ? k0 / 8 : H / 8;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < Nout<H>::LD; ++i) {
     const int e = threadIdx.x + 256 * i;
     if (e < 32 * per_row) *reinterpret_cast<u32x4*>(&T[e / per_row][8 * (e % per_row)]) = r[i];
   }
 }
 
 // acc = W-tile (32 rows x 16 KS columns) x act[0..KS) for this wave's 32 samples.
-template <int KS>
-__device__ __forceinline__ void nout_mfma(const uint16_t (*T)[kNoutTS], const bf16x8 (&act)[16],
-                                          f32x16& acc) {
+template <int H, int KS>
+__device__ __forceinline__ void nout_mfma(const uint16_t (*T)[Nout<H>::TS],
+                                          const bf16x8 (&act)[Nout<H>::KS], f32x16& acc) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
@@ -581,25 +421,27 @@ __device__ __forceinline__ void nout_epilogue(const float* bias, int nb, const f
   hi_frag = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
 }
 
-// One layer of 8 output blocks (KS k16-steps of input); t advances by 8 (a layer
-// starts at an even t, so the parity of t is that of the block).  Tile t + 1 is
-// in registers (r1 for odd t + 1, r0 for even) when block t starts: its global
-// loads were issued a whole block earlier; block t issues tile t + 2's.
-template <int KS>
+// One layer of NB output blocks (KS k16-steps of input); t advances by NB (a
+// layer starts at an even t, so the parity of t is that of the block).  Tile
+// t + 1 is in registers (r1 for odd t + 1, r0 for even) when block t starts: its
+// global loads were issued a whole block earlier; block t issues tile t + 2's.
+template <int H, int KS>
 __device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const float* bias,
-                                           uint16_t (*tile)[32][kNoutTS], int& t, int total,
-                                           const bf16x8 (&act)[16], bf16x8 (&nxt)[16],
-                                           u32x4 (&r0)[4], u32x4 (&r1)[4]) {
+                                           uint16_t (*tile)[32][Nout<H>::TS], int& t, int total,
+                                           const bf16x8 (&act)[Nout<H>::KS],
+                                           bf16x8 (&nxt)[Nout<H>::KS], u32x4 (&r0)[Nout<H>::LD],
+                                           u32x4 (&r1)[Nout<H>::LD]) {
+  constexpr int NB = Nout<H>::NB;
   f32x16 prev;
-  static_for<0, 8>([&](auto NB) {
-    constexpr int nb = decltype(NB)::value;
-    auto block = [&](u32x4 (&next)[4], u32x4 (&after)[4]) {  // tiles t + 1, t + 2
+  static_for<0, NB>([&](auto Bc) {
+    constexpr int nb = decltype(Bc)::value;
+    auto block = [&](u32x4 (&next)[Nout<H>::LD], u32x4 (&after)[Nout<H>::LD]) {  // t + 1, t + 2
       __syncthreads();  // tile t staged; the other buffer is free
-      if (t + 2 < total) nout_fetch(a, t + 2, k0, after);
+      if (t + 2 < total) nout_fetch<H>(a, t + 2, k0, after);
       f32x16 cur;
-      nout_mfma<KS>(tile[t & 1], act, cur);
+      nout_mfma<H, KS>(tile[t & 1], act, cur);
       if constexpr (nb > 0) nout_epilogue(bias, nb - 1, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
-      if (t + 1 < total) nout_stage(a, t + 1, k0, tile[(t + 1) & 1], next);
+      if (t + 1 < total) nout_stage<H>(t + 1, k0, tile[(t + 1) & 1], next);
       prev = cur;
       ++t;
     };
@@ -608,18 +450,20 @@ __device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const flo
     else
       block(r1, r0);
   });
-  nout_epilogue(bias, 7, prev, nxt[14], nxt[15]);
+  nout_epilogue(bias, NB - 1, prev, nxt[2 * NB - 2], nxt[2 * NB - 1]);
 }
 
-template <int K0S>
-__global__ void __launch_bounds__(256, 2) mlp_fused_forward_n256(const FusedArgs a) {
-  __shared__ __align__(16) uint16_t tile[2][32][kNoutTS];
+// K0S = k16 steps of the padded input width (round64(w0) / 16).
+template <int H, int K0S>
+__global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward_nout(const FusedArgs a) {
+  using NO = Nout<H>;
+  __shared__ __align__(16) uint16_t tile[2][32][NO::TS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
   const int64_t m = static_cast<int64_t>(blockIdx.x) * 128 + wave * 32 + ln;  // this lane's sample
   constexpr int k0 = 16 * K0S;
-  bf16x8 act[16], nxt[16];
+  bf16x8 act[NO::KS], nxt[NO::KS];
 #pragma unroll
-  for (int ks = 0; ks < 16; ++ks) {
+  for (int ks = 0; ks < NO::KS; ++ks) {
     uint32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -633,21 +477,21 @@ __global__ void __launch_bounds__(256, 2) mlp_fused_forward_n256(const FusedArgs
     }
     act[ks] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
   }
-  const int total = 8 * (a.layers - 1) + 1;
+  const int total = NO::NB * (a.layers - 1) + 1;
   int t = 0;
-  u32x4 r0[4], r1[4];
-  nout_fetch(a, 0, k0, r0);
-  nout_stage(a, 0, k0, tile[0], r0);
-  nout_fetch(a, 1, k0, r1);  // tile 1 in flight
-  nout_layer<K0S>(a, k0, a.b[0], tile, t, total, act, nxt, r0, r1);
+  u32x4 r0[NO::LD], r1[NO::LD];
+  nout_fetch<H>(a, 0, k0, r0);
+  nout_stage<H>(0, k0, tile[0], r0);
+  nout_fetch<H>(a, 1, k0, r1);  // tile 1 in flight
+  nout_layer<H, K0S>(a, k0, a.b[0], tile, t, total, act, nxt, r0, r1);
   for (int l = 1; l + 1 < a.layers; ++l) {
 #pragma unroll
-    for (int ks = 0; ks < 16; ++ks) act[ks] = nxt[ks];
-    nout_layer<16>(a, k0, a.b[l], tile, t, total, act, nxt, r0, r1);
+    for (int ks = 0; ks < NO::KS; ++ks) act[ks] = nxt[ks];
+    nout_layer<H, NO::KS>(a, k0, a.b[l], tile, t, total, act, nxt, r0, r1);
   }
   __syncthreads();  // output layer tile staged
   f32x16 out;
-  nout_mfma<16>(tile[t & 1], nxt, out);
+  nout_mfma<H, NO::KS>(tile[t & 1], nxt, out);
   const float* bias = a.b[a.layers - 1];
   if (m < a.rows) {
 #pragma unroll
@@ -1213,13 +1057,14 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     a.W = reinterpret_cast<const uint16_t* const*>(m->ptrs);
     a.b = reinterpret_cast<const float* const*>(m->ptrs + kFusedMaxLayers);
     a.y = y;
-    if (m->fused_h == 256) {  // 32 samples per wave, output-block-major
-      const unsigned grid = static_cast<unsigned>((B + 127) / 128);
-      hipLaunchKernelGGL((mlp_fused_forward_n256<4>), dim3(grid), dim3(256), 0, s, a);
-    } else {  // 32 samples per wave, inputs in four 32-wide tiles
-      const unsigned grid = static_cast<unsigned>((B + 127) / 128);
-      hipLaunchKernelGGL((mlp_fused_forward<512, 1, 4>), dim3(grid), dim3(256), 0, s, a);
-    }
+    // 32 samples per wave, 4 waves per workgroup, output-block-major
+    const unsigned grid = static_cast<unsigned>((B + 127) / 128);
+    if (m->fused_h == 256)
+      hipLaunchKernelGGL((mlp_fused_forward_nout<256, 4>), dim3(grid), dim3(256), 0, s, a);
+    else if (round_up(a.w0, 64) == 64)
+      hipLaunchKernelGGL((mlp_fused_forward_nout<512, 4>), dim3(grid), dim3(256), 0, s, a);
+    else
+      hipLaunchKernelGGL((mlp_fused_forward_nout<512, 8>), dim3(grid), dim3(256), 0, s, a);
     MLP_HIP(hipGetLastError());
     return CE_OK;
   }

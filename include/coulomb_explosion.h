@@ -275,7 +275,7 @@ int ce_mlp_set_params(ce_mlp* m, const float* params);  /* also resets Adam */
 /* y [batch][w_last] = net(x [batch][w0]); host buffers. */
 int ce_mlp_forward(ce_mlp* m, const float* x, size_t batch, float* y);
 /* Inference (ce_mlp_forward and the closure) runs networks shaped
- * [w0 <= 96, H, ..., H, w_out <= 32] with H = 256 or 512 in one fused kernel
+ * [w0, H, ..., H, w_out <= 32] with H = 256 (w0 <= 64) or 512 (w0 <= 128) in one fused kernel
  * (activations kept on chip; same bits as the layer-by-layer GEMMs) unless
  * disabled here.  Returns 1 if the fused kernel is in use, 0 if not, < 0 on error. */
 int ce_mlp_set_fused(ce_mlp* m, int32_t enable);

@@ -107,3 +107,25 @@ def test_other_size_sets_run_per_size(gpu, oracle):
             # sweep reports 7.4e-4 at N=10, docs/benchmarks/0004-precision-sweep.md:
             # 83-88), so rsqrt-vs-1/sqrt rounding can move a rare sim past 1e-5.
             assert np.mean(err < 1e-5) > 0.99 and err.max() < 1e-4
+
+
+@pytest.mark.parametrize("co2_charges", [[1.0, 1.0, 1.0], [2.0, 1.0, 1.0]])
+def test_mixed_fixed_charges_vs_oracle(gpu, oracle, co2_charges):
+    """Fixed charges in the one-launch mixed kernel: all +1 runs the
+    uniform-pair-constant form for every species; one CO2 with a +2 carbon puts
+    the whole launch on the pair table.  Against the oracle's f32 lane form."""
+    ce = gpu
+    from oracle import ffi
+    sp = []
+    for (masses, R), q in zip(MIXED, [[1.0] * 2, co2_charges, [1.0] * 5, [1.0] * 8]):
+        mol = ce.Molecule([ce.Atom("X", m, c) for m, c in zip(masses, q)])
+        sp.append(ce.Species(mol, 500, R, 0.9))
+    mb = ce.MixedBatch(sp, 1.0, 1e-6, 1e-9)
+    mb.sample(seed=31)
+    res = mb.run(ce.RunConfig(**RULE))
+    assert mb.stats()["launches"] == 1
+    for s, g, r in zip(sp, mb.geometries(), res):
+        ref = oracle.lane_run(s.molecule.masses, s.molecule.charges, g,
+                              ffi.make_config(rtol=1e-6, atol=1e-9, **RULE))
+        assert per_atom_rel(r["momenta"], ref.momenta).max() < 1e-5
+        np.testing.assert_array_equal(r["status"], ref.status)

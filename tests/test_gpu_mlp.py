@@ -146,6 +146,7 @@ def test_mlp_argument_errors(gpu):
 
 @pytest.mark.parametrize("widths,rows", [([40, 256, 256, 256, 15], 1000),
                                          ([80, 512, 512, 512, 512, 30], 300),
+                                         ([50, 512, 512, 512, 9], 200),
                                          ([13, 256, 256, 6], 257)])
 def test_fused_forward_same_bits_as_layer_by_layer(gpu, widths, rows):
     """The fused inference kernel accumulates in the GEMM's K order: same bits."""
@@ -167,4 +168,5 @@ def test_fused_forward_same_bits_as_layer_by_layer(gpu, widths, rows):
 def test_fused_not_used_for_other_shapes(gpu):
     ce = gpu
     assert not ce.MLP([7, 64, 50, 3]).set_fused(True)
-    assert not ce.MLP([100, 256, 256, 8]).set_fused(True)  # input wider than 96
+    assert not ce.MLP([100, 256, 256, 8]).set_fused(True)  # input wider than 64 at H = 256
+    assert not ce.MLP([130, 512, 512, 8]).set_fused(True)  # input wider than 128 at H = 512
