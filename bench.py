@@ -502,7 +502,8 @@ def main() -> int:
         cpu = cpu_baseline(args.cpu_sims, args.cpu_reps, threads)
 
     kernel_name = ("dp5_persistent<float,5>" if args.workload == "ch4" else
-                   "dp5_mixed<float,2,3,5,8,per-sim charges>" if args.workload == "mixed" else
+                   "dp5_persistent<float,N,per-sim charges>, N = 2, 3, 5, 8 (one launch each)"
+                   if args.workload == "mixed" else
                    "dp5_persistent<float,10,per-sim charges>")
     line = {
         "metric": METRIC,

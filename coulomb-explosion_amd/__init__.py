@@ -93,7 +93,11 @@ class _ClosureStats(C.Structure):
 
 class _LaunchConfig(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("blocks_per_cu", C.c_int32),
-                ("refill_threshold", C.c_int32), ("chunk", C.c_int32)]
+                ("refill_threshold", C.c_int32), ("chunk", C.c_int32),
+                ("mixed_launch", C.c_int32)]
+
+
+MIXED_AUTO, MIXED_ONE_LAUNCH, MIXED_PER_SPECIES = 0, 1, 2
 
 
 def _load_library(path: str = LIB_PATH) -> C.CDLL:
@@ -363,8 +367,9 @@ class BatchedRK45:
         return out
 
     # -- configuration ------------------------------------------------------------
-    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0) -> None:
-        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk)
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0,
+                   mixed_launch=MIXED_AUTO) -> None:
+        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk, mixed_launch)
         _check(_lib.ce_engine_set_launch(self._h, C.byref(lc)))
 
     def launch(self) -> dict:
@@ -591,8 +596,9 @@ class MixedBatch:
         _check(_lib.ce_engine_get_charges(self._h, flat.ctypes.data, self.total))
         return self._split(flat, 1)
 
-    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0) -> None:
-        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk)
+    def set_launch(self, block_threads=0, blocks_per_cu=0, refill_threshold=0, chunk=0,
+                   mixed_launch=MIXED_AUTO) -> None:
+        lc = _LaunchConfig(block_threads, blocks_per_cu, refill_threshold, chunk, mixed_launch)
         _check(_lib.ce_engine_set_launch(self._h, C.byref(lc)))
 
     def stats(self) -> dict:

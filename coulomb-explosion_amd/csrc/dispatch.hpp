@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "coulomb_explosion.h"
 #include "kernels.hpp"
 
 namespace ce {
@@ -24,6 +25,7 @@ struct LaunchReq {
   bool per_sim_charges;
   int block_threads;               // 0 = 256
   int blocks_per_cu;               // 0 = occupancy of the kernel
+  int mixed_launch;                // CE_MIXED_* for sizes (2, 3, 5, 8)
   int n_cus;
   hipStream_t stream;
 };
@@ -136,15 +138,23 @@ hipError_t launch_mixed(const LaunchReq& req, LaunchInfo* info) {
   return hipGetLastError();
 }
 
-// The mixed launch compiled for sizes (2, 3, 5, 8) in that order; any other
-// combination runs one launch per segment on the same stream.  Without per-sim
+// The mixed launch compiled for sizes (2, 3, 5, 8) in that order (as
+// req.mixed_launch selects); any other combination runs one launch per segment on
+// the same stream.  Without per-sim
 // charges and with UNIFORM_OK (the f32 path), a launch whose species all have a
 // single pair constant runs the kUniform kernel.
 template <class T, bool PS, bool UNIFORM_OK>
 hipError_t launch_all(const LaunchReq& req, LaunchInfo* info, bool* bad_n) {
   const RunArgs& a = req.args;
-  const bool mixed_2358 = a.n_segments == 4 && req.species[0].n == 2 && req.species[1].n == 3 &&
-                          req.species[2].n == 5 && req.species[3].n == 8;
+  bool mixed_2358 = a.n_segments == 4 && req.species[0].n == 2 && req.species[1].n == 3 &&
+                    req.species[2].n == 5 && req.species[3].n == 8;
+  if (mixed_2358 && req.mixed_launch != CE_MIXED_ONE_LAUNCH) {
+    // Per-species launches give the small molecules their own occupancy (the one
+    // launch carries the N = 8 segment's registers) at the cost of a tail per launch.
+    bool large = true;
+    for (int s = 0; s < 4; ++s) large = large && a.seg[s].count >= (uint64_t(1) << 19);
+    mixed_2358 = req.mixed_launch == CE_MIXED_AUTO && !large;
+  }
   if (mixed_2358) {
     if constexpr (!PS && UNIFORM_OK) {
       bool all_uniform = true;

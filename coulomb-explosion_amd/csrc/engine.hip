@@ -543,6 +543,8 @@ int ce_engine_set_launch(ce_engine* e, const ce_launch_config* lc) {
     return fail(CE_ERR_INVALID_ARGUMENT, "refill_threshold must be in [1, 64]");
   if (lc->chunk != 0 && (lc->chunk < 64 || lc->chunk > (1 << 20)))
     return fail(CE_ERR_INVALID_ARGUMENT, "chunk must be in [64, 2^20]");
+  if (lc->mixed_launch < CE_MIXED_AUTO || lc->mixed_launch > CE_MIXED_PER_SPECIES)
+    return fail(CE_ERR_INVALID_ARGUMENT, "mixed_launch must be CE_MIXED_AUTO, _ONE_LAUNCH or _PER_SPECIES");
   e->launch = *lc;
   if (e->launch.refill_threshold == 0) e->launch.refill_threshold = 4;
   if (e->launch.chunk == 0) e->launch.chunk = 64;
@@ -709,6 +711,7 @@ int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, si
   req->per_sim_charges = e->have_charges;
   req->block_threads = e->launch.block_threads;
   req->blocks_per_cu = e->launch.blocks_per_cu;
+  req->mixed_launch = e->launch.mixed_launch;
   req->n_cus = e->n_cus;
   req->stream = e->stream;
   return CE_OK;

@@ -157,12 +157,21 @@ typedef struct {
 
 /* Launch shape: 0 fields mean "engine default".  refill_threshold = idle lanes a
  * wave waits for before it pulls new sims (1..64); chunk = sims a wave claims
- * from the device work queue per atomic. */
+ * from the device work queue per atomic.  mixed_launch, for a mixed engine of
+ * sizes (2, 3, 5, 8): CE_MIXED_ONE_LAUNCH runs all species in one persistent
+ * kernel (compiled for the largest size's registers: one wave per SIMD),
+ * CE_MIXED_PER_SPECIES one kernel per species on the engine stream (each at its
+ * own occupancy, one tail each), CE_MIXED_AUTO (0) the latter when every species
+ * has at least 2^19 sims, else the former. */
+#define CE_MIXED_AUTO 0
+#define CE_MIXED_ONE_LAUNCH 1
+#define CE_MIXED_PER_SPECIES 2
 typedef struct {
-  int32_t block_threads;     /* multiple of 64, <= 1024 */
+  int32_t block_threads;     /* multiple of 64, <= 256 */
   int32_t blocks_per_cu;     /* resident blocks per CU to launch */
   int32_t refill_threshold;
   int32_t chunk;
+  int32_t mixed_launch;      /* CE_MIXED_* */
 } ce_launch_config;
 
 typedef struct ce_engine ce_engine;

@@ -6,6 +6,7 @@ import ctypes as C
 import json
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -34,12 +35,35 @@ def test_library_is_built_for_gfx950(ce):
     assert b"gfx950" in data
 
 
-def test_struct_layouts_match_header(ce):
-    """ctypes mirrors of the header structs: sizes as the C compiler lays them out."""
-    assert C.sizeof(ce._MoleculeDesc) == 32
-    assert C.sizeof(ce._RunConfig) == 5 * 8 + 8 + 6 * 4
-    assert C.sizeof(ce._Results) == 8 * 8 + 8
-    assert C.sizeof(ce._LaunchConfig) == 16
+STRUCTS = {"ce_molecule_desc": "_MoleculeDesc", "ce_tolerances": "_Tolerances",
+           "ce_run_config": "_RunConfig", "ce_results": "_Results", "ce_run_stats": "_RunStats",
+           "ce_species_desc": "_SpeciesDesc", "ce_closure_stats": "_ClosureStats",
+           "ce_launch_config": "_LaunchConfig"}
+
+
+def test_struct_layouts_match_header(ce, tmp_path):
+    """ctypes mirrors of the header structs: size and every field's offset as the C
+    compiler lays the header out (a small C program built with gcc prints them)."""
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "coulomb_explosion.h"',
+             'int main(void) {']
+    for cname, pyname in STRUCTS.items():
+        st = getattr(ce, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in st._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True, capture_output=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(ln.split()[:2]): int(ln.split()[2]) for ln in out if ln.strip()}
+    for cname, pyname in STRUCTS.items():
+        st = getattr(ce, pyname)
+        assert C.sizeof(st) == got[(cname, "size")], cname
+        for f, _ in st._fields_:
+            assert getattr(st, f).offset == got[(cname, f)], (cname, f)
 
 
 def test_run_config_defaults_are_the_reference_defaults(ce):

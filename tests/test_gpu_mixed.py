@@ -129,3 +129,27 @@ def test_mixed_fixed_charges_vs_oracle(gpu, oracle, co2_charges):
                               ffi.make_config(rtol=1e-6, atol=1e-9, **RULE))
         assert per_atom_rel(r["momenta"], ref.momenta).max() < 1e-5
         np.testing.assert_array_equal(r["status"], ref.status)
+
+
+def test_mixed_one_launch_and_per_species_launches_agree(gpu):
+    """Sizes (2, 3, 5, 8) run as one persistent kernel or as one kernel per species
+    (the automatic choice for >= 2^19 sims per species): the same bits per sim."""
+    ce = gpu
+    sp = species(ce, 1 << 19)
+    mb = ce.MixedBatch(sp, 1.0, 1e-6, 1e-9)
+    mb.sample(seed=17)
+    cfg = ce.RunConfig(**RULE)
+    mb.run(cfg, download=False)
+    assert mb.stats()["launches"] == 4  # automatic: every species is large
+    mb.set_launch(mixed_launch=ce.MIXED_ONE_LAUNCH)
+    one = mb.run(cfg)
+    assert mb.stats()["launches"] == 1
+    mb.set_launch(mixed_launch=ce.MIXED_PER_SPECIES)
+    per = mb.run(cfg)
+    assert mb.stats()["launches"] == 4
+    for a, b in zip(one, per):
+        np.testing.assert_array_equal(a["momenta"], b["momenta"])
+        np.testing.assert_array_equal(a["steps"], b["steps"])
+        np.testing.assert_array_equal(a["status"], b["status"])
+    with pytest.raises(ValueError):
+        mb.set_launch(mixed_launch=3)
