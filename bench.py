@@ -397,6 +397,9 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--block-threads", type=int, default=0)
+    ap.add_argument("--mixed-launch", choices=["one", "per-species"], default="one",
+                    help="mixed: all sizes in one persistent launch (the workload as stated), "
+                         "or one launch per size")
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch shape on this workload first (cached)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -449,13 +452,15 @@ def main() -> int:
         tuned = entry["shape"]
     else:
         tuned = autotune.apply_cached(ce, w.eng, cfg, key_extra=args.workload)
-    if any((args.refill_threshold, args.chunk, args.blocks_per_cu, args.block_threads)):
-        shape = dict(tuned or {})
-        for k, v in (("refill_threshold", args.refill_threshold), ("chunk", args.chunk),
-                     ("blocks_per_cu", args.blocks_per_cu), ("block_threads", args.block_threads)):
-            if v:
-                shape[k] = v
-        w.eng.set_launch(**shape)
+    shape = dict(tuned or {})
+    for k, v in (("refill_threshold", args.refill_threshold), ("chunk", args.chunk),
+                 ("blocks_per_cu", args.blocks_per_cu), ("block_threads", args.block_threads)):
+        if v:
+            shape[k] = v
+    if args.workload == "mixed":
+        shape["mixed_launch"] = (ce.MIXED_ONE_LAUNCH if args.mixed_launch == "one"
+                                 else ce.MIXED_PER_SPECIES)
+    w.eng.set_launch(**shape)
 
     print(f"[bench] {args.workload}: warmup {args.warmup} x {args.sims} sims", file=sys.stderr,
           flush=True)
@@ -489,6 +494,20 @@ def main() -> int:
     achieved_tflops = flops_launch / avg_kernel_s / 1e12
     bytes_launch = sum(cnt * (24 * n + 24 * n + 12 * n + 25) for n, cnt in w.sizes)
 
+    other = None
+    if args.workload == "mixed":
+        # the other launch mode on the same workload, for comparison (not the value)
+        alt = "per-species" if args.mixed_launch == "one" else "one"
+        w.eng.set_launch(**dict(shape, mixed_launch=ce.MIXED_PER_SPECIES if alt == "per-species"
+                                else ce.MIXED_ONE_LAUNCH))
+        alt_ms = []
+        for _ in range(3):
+            w.sample()
+            w.run(cfg)
+            alt_ms.append(w.eng.stats()["kernel_ms"])
+        other = {"mode": alt, "kernel_ms_avg": float(np.mean(alt_ms[1:])),
+                 "launches": w.eng.stats()["launches"]}
+
     acc = None
     if args.accuracy_subset > 0:
         print(f"[bench] accuracy subset {args.accuracy_subset}", file=sys.stderr, flush=True)
@@ -502,7 +521,8 @@ def main() -> int:
         cpu = cpu_baseline(args.cpu_sims, args.cpu_reps, threads)
 
     kernel_name = ("dp5_persistent<float,5>" if args.workload == "ch4" else
-                   "dp5_persistent<float,N,per-sim charges>, N = 2, 3, 5, 8 (one launch each)"
+                   ("dp5_mixed<float,2,3,5,8,per-sim charges>" if args.mixed_launch == "one" else
+                    "dp5_persistent<float,N,per-sim charges>, N = 2, 3, 5, 8 (one launch each)")
                    if args.workload == "mixed" else
                    "dp5_persistent<float,10,per-sim charges>")
     line = {
@@ -519,7 +539,10 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (device counter-based uniform-sphere sampler, seed %d)" % SEED,
         "config": {
-            "workload": w.describe(),
+            "workload": (w.describe() if not (args.workload == "mixed" and
+                                               args.mixed_launch == "per-species")
+                         else w.describe().replace("in one persistent launch",
+                                                   "in one persistent launch per size")),
             "sims_per_step": args.sims,
             "method": "plain DP5(4), per-lane dt, device work-queue refill",
             "converged_per_step": n_ok,
@@ -532,6 +555,8 @@ def main() -> int:
             "kernel_ms_avg": avg_kernel_s * 1e3,
             "sampler_ms_avg": float(np.mean(sampler_ms)),
             "launches_per_step": launches,
+            **({"mixed_launch": args.mixed_launch, "other_mixed_launch": other}
+               if args.workload == "mixed" else {}),
             "launch": {"grid": stats["grid_blocks"], "block": stats["block_threads"],
                        "cus": stats["n_cus"], **(tuned or {})},
         },
