@@ -494,6 +494,23 @@ def main() -> int:
     achieved_tflops = flops_launch / avg_kernel_s / 1e12
     bytes_launch = sum(cnt * (24 * n + 24 * n + 12 * n + 25) for n, cnt in w.sizes)
 
+    writeback = None
+    if args.workload == "ch4":
+        # Dataset records of the last batch (examples/generate_dataset.cpp:198-217): the
+        # pack kernel reads geometry f64, momenta f32, steps, status (185 B/sim) and
+        # writes 125 B/sim packed; then the records are copied to host memory.
+        t1 = time.perf_counter()
+        rec = w.eng.records()
+        wall = time.perf_counter() - t1
+        pack_ms = w.eng.stats()["pack_ms"]
+        n_rec = len(rec)
+        writeback = {"records": n_rec, "record_bytes": int(rec.itemsize),
+                     "pack_kernel_ms": pack_ms,
+                     "pack_hbm_gbs": n_rec * (185 + rec.itemsize) / (pack_ms * 1e-3) / 1e9,
+                     "host_copy_gbs_incl_pcie": n_rec * rec.itemsize / max(wall - pack_ms * 1e-3,
+                                                                          1e-9) / 1e9}
+        del rec
+
     other = None
     if args.workload == "mixed":
         # the other launch mode on the same workload, for comparison (not the value)
@@ -557,6 +574,7 @@ def main() -> int:
             "launches_per_step": launches,
             **({"mixed_launch": args.mixed_launch, "other_mixed_launch": other}
                if args.workload == "mixed" else {}),
+            **({"record_writeback": writeback} if writeback else {}),
             "launch": {"grid": stats["grid_blocks"], "block": stats["block_threads"],
                        "cus": stats["n_cus"], **(tuned or {})},
         },
