@@ -167,3 +167,36 @@ def test_pe_rule_reference_defaults_vs_ref_engine(gpu, oracle):
                             ffi.make_config(rtol=1e-6, atol=1e-9, **cfg), precision="f64")
     assert (res.status == ce.SIM_OK_PE).all()
     assert per_atom_rel(res.momenta, ref.momenta).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", list(range(2, 11)))
+@pytest.mark.parametrize("uniform", [True, False])
+def test_every_molecule_size_vs_oracle_lane(gpu, oracle, n, uniform):
+    """Every compiled molecule size, with equal charges (the uniform-pair-constant
+    kernel) and with unequal ones (the pair table): f32 engine vs the oracle's f32
+    lane form (<= 1e-5) and f64 engine vs the f64 lane form (identical steps,
+    <= 1e-12), under the headline rule."""
+    ce = gpu
+    from oracle import ffi
+    rng = np.random.default_rng(100 * n + uniform)
+    masses = list(rng.choice([1837.15, 21874.66, 25526.04, 29156.95], size=n))
+    charges = [1.0] * n if uniform else list(rng.integers(1, 3, size=n).astype(float))
+    if not uniform and n > 2 and len(set(charges)) == 1:
+        charges[0] = 3.0 - charges[0]
+    mol = ce.Molecule([ce.Atom("X", m, q) for m, q in zip(masses, charges)])
+    _, geom = oracle.sample(7 + n, n, 400, n ** (1 / 3) + 1.5, 0.9)
+    for precision, rtol, atol in ((0, 1e-6, 1e-9), (1, 1e-10, 1e-14)):
+        _, res = engine_run(ce, mol, geom, rtol, atol, precision=precision, **HEADLINE_RULE)
+        ref = oracle.lane_run(masses, charges, geom,
+                              ffi.make_config(rtol=rtol, atol=atol, **HEADLINE_RULE),
+                              precision="f32" if precision == 0 else "f64")
+        err = per_atom_rel(res.momenta, ref.momenta)
+        if precision == 0:
+            # v_rsq / v_rcp against 1/sqrt and 1/x: an odd sim takes a step more or
+            # less; with charges up to 2 and 8-10 atoms f32 itself is only good to
+            # ~1e-4 on such sims (the same bar as the mixed-size tests)
+            assert np.mean(err < 1e-5) >= 0.99 and err.max() < 1e-4, err.max()
+            assert np.mean(res.steps == ref.steps) > 0.8
+        else:
+            assert err.max() < 1e-12, err.max()
+            np.testing.assert_array_equal(res.steps, ref.steps)
