@@ -321,12 +321,29 @@ struct FusedArgs {
   int w0;
   int layers;      // weight layers (>= 2): w0->H, (layers - 2) x H->H, H->w_out
   int w_out;
-  // Per-layer pointers in device memory (indexing a kernel-argument array with a
-  // run-time layer would copy the arguments to scratch):
-  const uint16_t* const* W;  // padded bf16 [round32(out)][round64(in)]
-  const float* const* b;     // [out]
+  // Padded bf16 weights of all layers back to back: [H][k0] (k0 = round64(w0)),
+  // then [H][H] per hidden layer, then [32][H]; layer l >= 1 starts at
+  // W + H k0 + (l - 1) H H (uniform arithmetic, no per-block table lookup).
+  const uint16_t* W;
+  // Per-layer bias pointers, read only through layer_b (constant indices:
+  // indexing a kernel-argument array with a run-time layer would copy the
+  // arguments to scratch).
+  const float* b[kFusedMaxLayers];     // [out]
   float* y;                            // [rows][w_out]
 };
+
+__device__ __forceinline__ const float* layer_b(const FusedArgs& a, int l) {
+  switch (l) {
+    case 0: return a.b[0];
+    case 1: return a.b[1];
+    case 2: return a.b[2];
+    case 3: return a.b[3];
+    case 4: return a.b[4];
+    case 5: return a.b[5];
+    case 6: return a.b[6];
+    default: return a.b[7];
+  }
+}
 
 // Output-block-major fused inference (H = 256 or 512): the weights stream through
 // LDS one 32-row output block (all K columns) at a time, so a block's accumulators
@@ -334,6 +351,19 @@ struct FusedArgs {
 // MFMAs.  Each wave holds 32 samples: the input fragments (act) and the next
 // layer's fragments (nxt) in registers, H / 4 VGPRs each -- two waves per SIMD at
 // H = 256 (their MFMA and VALU work also overlap), one at H = 512.
+// Loads through pointers read from the device pointer table: cast to the global
+// address space, so they are global loads (vmcnt only) rather than flat loads,
+// whose lgkmcnt share would make every LDS wait also wait for the weight prefetch.
+typedef const u32x4 __attribute__((address_space(1))) global_u32x4;
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef const f32x4 __attribute__((address_space(1))) global_f32x4;
+__device__ __forceinline__ u32x4 load_global16(const void* p) {
+  return *(global_u32x4*)(p);
+}
+__device__ __forceinline__ f32x4 load_global_f32x4(const float* p) {
+  return *(global_f32x4*)(p);
+}
+
 template <int H>
 struct Nout {
   static constexpr int TS = H + 8;         // LDS tile row stride (elements)
@@ -349,28 +379,31 @@ __device__ __forceinline__ void nout_tile(const FusedArgs& a, int t, int k0, con
                                           int* ldw, int* row0) {
   constexpr int NB = Nout<H>::NB;
   if (t < NB) {
-    *W = a.W[0];
+    *W = a.W;
     *ldw = k0;
     *row0 = 32 * t;
   } else {
     const int l = 1 + (t - NB) / NB;
-    *W = a.W[l];
+    *W = a.W + static_cast<int64_t>(H) * k0 + static_cast<int64_t>(l - 1) * H * H;
     *ldw = H;
     *row0 = l == a.layers - 1 ? 0 : 32 * ((t - NB) % NB);
   }
 }
 
+// Tile t's weights into registers; past the last tile the last one is loaded
+// again, so every block issues the same loads and the compiler's wait counts stay
+// exact (a load issued on only some paths makes later waits conservative).
 template <int H>
-__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int k0,
+__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int total, int k0,
                                            u32x4 (&r)[Nout<H>::LD]) {
   const uint16_t* W;
   int ldw, row0;
-  nout_tile<H>(a, t, k0, &W, &ldw, &row0);
+  nout_tile<H>(a, t < total ? t : total - 1, k0, &W, &ldw, &row0);
   const int per_row = ldw / 8;  // 16-byte chunks per row
 #pragma unroll
   for (int i = 0; i < Nout<H>::LD; ++i) {  // every register written (chunks past the tile reload chunk 0)
     const int e = threadIdx.x + 256 * i < 32 * per_row ? threadIdx.x + 256 * i : 0;
-    r[i] = *reinterpret_cast<const u32x4*>(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
+    r[i] = load_global16(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
   }
 }
 
@@ -399,16 +432,22 @@ __device__ __forceinline__ void nout_mfma(const uint16_t (*T)[Nout<H>::TS],
   }
 }
 
-// SiLU(acc + bias) of output block nb -> the next layer's fragments 2 nb, 2 nb + 1.
-__device__ __forceinline__ void nout_epilogue(const float* bias, int nb, const f32x16& acc,
-                                              bf16x8& lo_frag, bf16x8& hi_frag) {
+// The 16 biases of output block nb this lane's accumulators need (features
+// 8q + 4h + u of the block).
+__device__ __forceinline__ void nout_bias(const float* bias, int nb, f32x4 (&bv)[4]) {
   const int h = (threadIdx.x & 63) >> 5;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) bv[q] = load_global_f32x4(bias + 32 * nb + 8 * q + 4 * h);
+}
+
+// SiLU(acc + bias) of an output block -> the next layer's fragments 2 nb, 2 nb + 1.
+__device__ __forceinline__ void nout_epilogue(const f32x4 (&bv)[4], const f32x16& acc,
+                                              bf16x8& lo_frag, bf16x8& hi_frag) {
   float v[16];
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      v[4 * q + u] = silu(acc[4 * q + u] + bias[32 * nb + 8 * q + 4 * h + u]);
+    for (int u = 0; u < 4; ++u) v[4 * q + u] = silu(acc[4 * q + u] + bv[q][u]);
   uint32_t P0 = pack_bf16(v[0], v[1]), P1 = pack_bf16(v[2], v[3]);
   uint32_t Q0 = pack_bf16(v[4], v[5]), Q1 = pack_bf16(v[6], v[7]);
   uint32_t R0 = pack_bf16(v[8], v[9]), R1 = pack_bf16(v[10], v[11]);
@@ -437,10 +476,14 @@ __device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const flo
     constexpr int nb = decltype(Bc)::value;
     auto block = [&](u32x4 (&next)[Nout<H>::LD], u32x4 (&after)[Nout<H>::LD]) {  // t + 1, t + 2
       __syncthreads();  // tile t staged; the other buffer is free
-      if (t + 2 < total) nout_fetch<H>(a, t + 2, k0, after);
+      // The previous block's biases are loaded before tile t + 2's weights, so
+      // waiting for them does not wait for the weight prefetch.
+      f32x4 bv[4];
+      if constexpr (nb > 0) nout_bias(bias, nb - 1, bv);
+      nout_fetch<H>(a, t + 2, total, k0, after);
       f32x16 cur;
       nout_mfma<H, KS>(tile[t & 1], act, cur);
-      if constexpr (nb > 0) nout_epilogue(bias, nb - 1, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
+      if constexpr (nb > 0) nout_epilogue(bv, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
       if (t + 1 < total) nout_stage<H>(t + 1, k0, tile[(t + 1) & 1], next);
       prev = cur;
       ++t;
@@ -450,7 +493,9 @@ __device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const flo
     else
       block(r1, r0);
   });
-  nout_epilogue(bias, NB - 1, prev, nxt[2 * NB - 2], nxt[2 * NB - 1]);
+  f32x4 bv[4];
+  nout_bias(bias, NB - 1, bv);
+  nout_epilogue(bv, prev, nxt[2 * NB - 2], nxt[2 * NB - 1]);
 }
 
 // K0S = k16 steps of the padded input width (round64(w0) / 16).
@@ -480,19 +525,19 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward_nout(
   const int total = NO::NB * (a.layers - 1) + 1;
   int t = 0;
   u32x4 r0[NO::LD], r1[NO::LD];
-  nout_fetch<H>(a, 0, k0, r0);
+  nout_fetch<H>(a, 0, total, k0, r0);
   nout_stage<H>(0, k0, tile[0], r0);
-  nout_fetch<H>(a, 1, k0, r1);  // tile 1 in flight
-  nout_layer<H, K0S>(a, k0, a.b[0], tile, t, total, act, nxt, r0, r1);
+  nout_fetch<H>(a, 1, total, k0, r1);  // tile 1 in flight
+  nout_layer<H, K0S>(a, k0, layer_b(a, 0), tile, t, total, act, nxt, r0, r1);
   for (int l = 1; l + 1 < a.layers; ++l) {
 #pragma unroll
     for (int ks = 0; ks < NO::KS; ++ks) act[ks] = nxt[ks];
-    nout_layer<H, NO::KS>(a, k0, a.b[l], tile, t, total, act, nxt, r0, r1);
+    nout_layer<H, NO::KS>(a, k0, layer_b(a, l), tile, t, total, act, nxt, r0, r1);
   }
   __syncthreads();  // output layer tile staged
   f32x16 out;
   nout_mfma<H, NO::KS>(tile[t & 1], nxt, out);
-  const float* bias = a.b[a.layers - 1];
+  const float* bias = layer_b(a, a.layers - 1);
   if (m < a.rows) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -526,8 +571,8 @@ struct ce_mlp {
   int fused_h = 0;            // hidden width of the fused inference kernel, 0 = none
   bool use_fused = true;      // inference (ce_mlp_forward, closure) through it
   bool wp_stale = true;       // Wp older than the master parameters
-  std::vector<uint16_t*> Wp;  // padded bf16 copies for the fused kernel
-  void** ptrs = nullptr;      // device table: Wp[0..L), then the bias pointers
+  uint16_t* Wp_all = nullptr;  // padded bf16 copies for the fused kernel, one allocation
+  std::vector<uint16_t*> Wp;   // layer l's copy inside Wp_all
   int64_t adam_t = 0;
   // batch workspace
   int64_t cap = 0;
@@ -574,8 +619,7 @@ void free_mlp(ce_mlp* m) {
   f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16);
   f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
   for (auto* p : m->Wb) f(p);
-  for (auto* p : m->Wp) f(p);
-  f(m->ptrs);
+  f(m->Wp_all);
   for (auto* p : m->act) f(p);
   for (auto* p : m->Z) f(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -802,21 +846,17 @@ int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp
     m->fused_h = ok ? h : 0;
   }
   if (m->fused_h) {
+    // Layers back to back: [H][round64(w0)], then (L - 2) x [H][H], then [32][H], so
+    // the kernel finds layer l >= 1 at a uniform stride from the first.
     m->Wp.assign(m->L, nullptr);
-    for (int l = 0; l < m->L && rc == CE_OK; ++l)
-      rc = dalloc(&m->Wp[l], static_cast<int64_t>(round_up(widths[l + 1], 32)) *
-                                 round_up(widths[l], 64));
-    if (rc == CE_OK) rc = dalloc(&m->ptrs, 2 * kFusedMaxLayers);
-    if (rc == CE_OK) {
-      std::vector<void*> table(2 * kFusedMaxLayers, nullptr);
-      for (int l = 0; l < m->L; ++l) {
-        table[l] = m->Wp[l];
-        table[kFusedMaxLayers + l] = m->P + m->b_off[l];
-      }
-      if (hipMemcpy(m->ptrs, table.data(), table.size() * sizeof(void*), hipMemcpyHostToDevice) !=
-          hipSuccess)
-        rc = fail(CE_ERR_HIP, "pointer table upload");
+    int64_t total = 0;
+    std::vector<int64_t> off(m->L);
+    for (int l = 0; l < m->L; ++l) {
+      off[l] = total;
+      total += static_cast<int64_t>(round_up(widths[l + 1], 32)) * round_up(widths[l], 64);
     }
+    if (rc == CE_OK) rc = dalloc(&m->Wp_all, total);
+    for (int l = 0; l < m->L && rc == CE_OK; ++l) m->Wp[l] = m->Wp_all + off[l];
   }
   if (rc == CE_OK) {
     std::vector<float> p;
@@ -1054,8 +1094,8 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     a.w0 = m->widths[0];
     a.layers = m->L;
     a.w_out = m->widths[m->L];
-    a.W = reinterpret_cast<const uint16_t* const*>(m->ptrs);
-    a.b = reinterpret_cast<const float* const*>(m->ptrs + kFusedMaxLayers);
+    a.W = m->Wp_all;
+    for (int l = 0; l < m->L; ++l) a.b[l] = m->P + m->b_off[l];
     a.y = y;
     // 32 samples per wave, 4 waves per workgroup, output-block-major
     const unsigned grid = static_cast<unsigned>((B + 127) / 128);
