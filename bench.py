@@ -405,7 +405,25 @@ def main() -> int:
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (HBM counter "
                          "calibration for rocprofv3 --pmc) and exit")
+    ap.add_argument("--mlp-probe", default="",
+                    help="only run the fused inference kernel: widths (e.g. 40,256,256,256,15), "
+                         "--sims rows of random inputs, 3 forward passes (counter runs), and exit")
     args = ap.parse_args()
+
+    if args.mlp_probe:
+        from __graft_entry__ import load_package
+        ce = load_package()
+        widths = [int(v) for v in args.mlp_probe.split(",")]
+        net = ce.MLP(widths, seed=1)
+        if not net.set_fused(True):
+            print("bench.py: these widths do not use the fused kernel", file=sys.stderr)
+            return 2
+        x = np.random.default_rng(0).normal(size=(args.sims, widths[0])).astype(np.float32)
+        for _ in range(3):
+            y = net.forward(x)
+        print(json.dumps({"mlp_probe": widths, "rows": args.sims, "passes": 3,
+                          "mean_abs_out": float(np.abs(y).mean())}))
+        return 0
 
     if args.copy_probe_mb:
         import torch
