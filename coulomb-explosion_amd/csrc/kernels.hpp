@@ -394,7 +394,8 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
   PairK<T, N, CM> pk{mc};
   T x[N3], v[N3], k[7][N3];
   T h = T(0);
-  double t = 0.0, e0 = 0.0, pe_stop = 0.0;
+  double t = 0.0;
+  T e0 = T(0);  // initial PE, computed in T (so exact in T), used in FP64
   uint32_t steps = 0, att = 0;
   int nrej = 0;
   bool rej_once = false;
@@ -420,7 +421,8 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         const double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
         ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
       }
-      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0 / ke) : 1.0;
+      const double e0d = static_cast<double>(e0);
+      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0d / ke) : 1.0;
       const uint64_t base = sg.pos_off + loc * N3;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -442,7 +444,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           sv[q] = v[q];
         }
         p.state_h[gs] = static_cast<double>(h);
-        p.state_e0[gs] = e0;
+        p.state_e0[gs] = e0d;
       }
       tot.att += p.resume ? att - p.attempts[gs] : att;
       p.steps[gs] = steps;
@@ -462,7 +464,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
             pe_end = fma(pk.template getd<i, j>(), rsqrt_f64(rx * rx + ry * ry + rz * rz), pe_end);
           });
         });
-        p.eerr[gs] = (e0 != 0.0) ? (ke + pe_end - e0) / e0 : 0.0;
+        p.eerr[gs] = (e0d != 0.0) ? (ke + pe_end - e0d) / e0d : 0.0;
       }
       if (p.trace_h != nullptr)
         for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
@@ -538,21 +540,20 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           att = p.attempts[gs];
           t = p.t_final[gs];
           h = static_cast<T>(p.state_h[gs]);
-          e0 = p.state_e0[gs];
-          pe_stop = p.pe_stop_fraction * e0;
+          e0 = static_cast<T>(p.state_e0[gs]);
           T unused;
           lane_force<T, N, CM, false, false>(x, k[0], unused, pk);
           if (steps >= p.max_steps) status = kSimNotConverged;
         } else if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
           status = kSimSamplerFail;
-          e0 = 0.0;
+          e0 = T(0);
         } else {
           T pe0;
           // The stages hold forces F = m a; 1/m enters via per-atom step factors.
           lane_force<T, N, CM, true, false>(x, k[0], pe0, pk);
-          e0 = static_cast<double>(pe0);
-          pe_stop = p.pe_stop_fraction * e0;
-          if (p.stop_rule == kStopPE && e0 <= pe_stop) {
+          e0 = pe0;
+          if (p.stop_rule == kStopPE &&
+              static_cast<double>(e0) <= p.pe_stop_fraction * static_cast<double>(e0)) {
             status = kSimOkPE;
           } else if (p.max_steps == 0) {
             status = kSimNotConverged;
@@ -585,12 +586,9 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       }
       const T hh = minT(h, cap);
       const T hh2 = hh * hh;
-      T g[N], gv[N];  // h^2 / m_i and h / m_i
+      T g[N];  // h^2 / m_i
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        g[i] = hh2 * mc.im[i];
-        gv[i] = hh * mc.im[i];
-      }
+      for (int i = 0; i < N; ++i) g[i] = hh2 * mc.im[i];
       ++att;
       T xs[N3], vn[N3];
       // Stages 2..6: positions x + h c_s v + (h^2/m) sum_l aa[s][l] F_l, then F(x_s).
@@ -610,30 +608,39 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         T unused;
         lane_force<T, N, CM, false, false>(xs, k[s], unused, pk);
       });
-      // Stage 7 (FSAL): the 5th-order solution; F(x_new) is the next k1.
+      // Stage 7 (FSAL): the 5th-order solution; F(x_new) is the next k1.  The
+      // embedded error in the RMS norm (scale = atol + rtol max(|y_old|, |y_new|),
+      // the squared norm compared with 1) is summed around F(x_new): its position
+      // part and the first six velocity terms need only k1..k6, which then die
+      // before the last force evaluation (k7 enters the velocity error alone,
+      // as the last FMA of the same ascending sum).
+      static_assert(kDP5.ea[6] == 0.0 && kDP5.e[6] != 0.0 && kDP5.e[0] != 0.0);
+      T exs = T(0), evs = T(0);
+      T evp[N3];
+      T gv[N];  // h / m_i
+#pragma unroll
+      for (int i = 0; i < N; ++i) gv[i] = hh * mc.im[i];
 #pragma unroll
       for (int q = 0; q < N3; ++q) {
         const T ax = stage_sum<T, 6>(k, q, CoefAA<6>{});
         const T av = stage_sum<T, 6>(k, q, CoefB{});
         xs[q] = fmaT(g[q / 3], ax, fmaT(hh, v[q], x[q]));
         vn[q] = fmaT(gv[q / 3], av, v[q]);
+        const T ex = stage_sum<T, 6>(k, q, CoefEA{});
+        evp[q] = stage_sum<T, 6>(k, q, CoefE{});
+        const T scx = fmaT(rtol, max_abs(x[q], xs[q]), atol);
+        const T rx = (g[q / 3] * ex) * rcpT(scx);
+        exs = fmaT(rx, rx, exs);
       }
       {
         T unused;
         lane_force<T, N, CM, false, false>(xs, k[6], unused, pk);
       }
-      // Embedded error in the RMS norm, scale = atol + rtol max(|y_old|, |y_new|);
-      // the squared norm is compared with 1.
-      T exs = T(0), evs = T(0);
 #pragma unroll
       for (int q = 0; q < N3; ++q) {
-        const T ex = stage_sum<T, 7>(k, q, CoefEA{});
-        const T ev = stage_sum<T, 7>(k, q, CoefE{});
-        const T scx = fmaT(rtol, max_abs(x[q], xs[q]), atol);
+        const T ev = fmaT(static_cast<T>(kDP5.e[6]), k[6][q], evp[q]);
         const T scv = fmaT(rtol, max_abs(v[q], vn[q]), atol);
-        const T rx = (g[q / 3] * ex) * rcpT(scx);
         const T rv = (gv[q / 3] * ev) * rcpT(scv);
-        exs = fmaT(rx, rx, exs);
         evs = fmaT(rv, rv, evs);
       }
       const T en2 = (exs + evs) * inv6n;
@@ -678,7 +685,9 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         ++tot.acc;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
-          if (static_cast<double>(lane_pe<T, N, CM>(x, pk)) <= pe_stop) status = kSimOkPE;
+          if (static_cast<double>(lane_pe<T, N, CM>(x, pk)) <=
+              p.pe_stop_fraction * static_cast<double>(e0))
+            status = kSimOkPE;
         } else if (settled) {
           status = kSimOkMomentum;
         } else if (final_step) {
