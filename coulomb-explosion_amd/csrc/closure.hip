@@ -473,6 +473,8 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
   if (!(gate_tol >= 0.0)) return fail(CE_ERR_INVALID_ARGUMENT, "gate_tol must be >= 0");
   if (!c->members.empty() && !c->normalized)
     return fail(CE_ERR_RUNTIME, "set the normalisation before running members");
+  if (e->loaded() > UINT32_MAX)
+    return fail(CE_ERR_INVALID_ARGUMENT, "the fallback list indexes at most 2^32 loaded sims");
   CL_HIP(hipSetDevice(c->device));
   rc = ce_engine_enable_state(e, 1);
   if (rc != CE_OK) return rc;

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -39,8 +40,17 @@ template <class T>
 int hip_alloc(T** p, size_t count) {
   *p = nullptr;
   if (count == 0) count = 1;
+  if (count > SIZE_MAX / sizeof(T))
+    return fail(CE_ERR_INVALID_ARGUMENT, "buffer size overflows size_t");
   CE_HIP(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
   return CE_OK;
+}
+
+// a * b + c without wrapping (false on overflow).
+bool mul_add_ok(size_t a, size_t b, size_t c, size_t* out) {
+  size_t m = 0;
+  if (__builtin_mul_overflow(a, b, &m)) return false;
+  return !__builtin_add_overflow(m, c, out);
 }
 
 }  // namespace
@@ -91,9 +101,11 @@ int init_engine(ce_engine* e, const ce_tolerances* tol) {
   e->total_pos = 0;
   e->total_q = 0;
   for (int s = 0; s < e->n_species; ++s) {
-    e->capacity += e->sp[s].capacity;
-    e->total_pos += e->sp[s].capacity * 3 * e->sp[s].n;
-    e->total_q += e->sp[s].capacity * e->sp[s].n;
+    const size_t cap = e->sp[s].capacity, n = static_cast<size_t>(e->sp[s].n);
+    if (!mul_add_ok(cap, 1, e->capacity, &e->capacity) ||
+        !mul_add_ok(cap, 3 * n, e->total_pos, &e->total_pos) ||
+        !mul_add_ok(cap, n, e->total_q, &e->total_q) || e->total_pos > SIZE_MAX / sizeof(double))
+      return fail(CE_ERR_INVALID_ARGUMENT, "capacity too large (buffer sizes overflow)");
   }
   hipDeviceProp_t prop;
   CE_HIP(hipGetDeviceProperties(&prop, e->device));
@@ -143,6 +155,8 @@ int check_config(const ce_run_config* c) {
     return fail(CE_ERR_INVALID_ARGUMENT, "fixed h0 must be positive");
   if (c->stop_rule == CE_STOP_MOMENTUM_OR_TMAX && !(c->t_max > 0.0))
     return fail(CE_ERR_INVALID_ARGUMENT, "t_max must be positive");
+  if (c->stop_rule == CE_STOP_MOMENTUM_OR_TMAX && !(c->momentum_tol >= 0.0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "momentum_tol must be >= 0");
   if (c->max_rejections < 0) return fail(CE_ERR_INVALID_ARGUMENT, "max_rejections < 0");
   return CE_OK;
 }
@@ -193,6 +207,13 @@ int copy_per_sim(ce_engine* e, void* host, const void* dev, size_t elem, hipMemc
 int copy_results(ce_engine* e, ce_results* out) {
   if (out == nullptr) return CE_OK;
   if (!e->have_results) return fail(CE_ERR_RUNTIME, "no results: run the engine first");
+  // Checked before any copy is queued, so an error leaves no copy in flight
+  // into the caller's buffers.
+  if (out->energy_err && !e->eerr_valid)
+    return fail(CE_ERR_INVALID_ARGUMENT,
+                "energy_err: the last run did not compute it (ce_engine_enable_energy_diagnostic)");
+  if (out->trace_h != nullptr && (out->trace_cap != e->trace_cap || e->trace == nullptr))
+    return fail(CE_ERR_INVALID_ARGUMENT, "trace_cap does not match the enabled trace");
   int rc = CE_OK;
   const hipMemcpyKind d2h = hipMemcpyDeviceToHost;
   if (rc == CE_OK && out->momenta) rc = copy_per_atom(e, out->momenta, e->mom, sizeof(double), 3, d2h);
@@ -201,19 +222,14 @@ int copy_results(ce_engine* e, ce_results* out) {
   if (rc == CE_OK && out->attempts) rc = copy_per_sim(e, out->attempts, e->attempts, sizeof(uint32_t), d2h);
   if (rc == CE_OK && out->status) rc = copy_per_sim(e, out->status, e->status, 1, d2h);
   if (rc == CE_OK && out->t_final) rc = copy_per_sim(e, out->t_final, e->tfin, sizeof(double), d2h);
-  if (rc == CE_OK && out->energy_err) {
-    if (!e->eerr_valid)
-      return fail(CE_ERR_INVALID_ARGUMENT,
-                  "energy_err: the last run did not compute it (ce_engine_enable_energy_diagnostic)");
+  if (rc == CE_OK && out->energy_err)
     rc = copy_per_sim(e, out->energy_err, e->eerr, sizeof(double), d2h);
-  }
-  if (rc == CE_OK && out->trace_h != nullptr) {
-    if (out->trace_cap != e->trace_cap || e->trace == nullptr)
-      return fail(CE_ERR_INVALID_ARGUMENT, "trace_cap does not match the enabled trace");
+  if (rc == CE_OK && out->trace_h != nullptr)
     rc = copy_per_sim(e, out->trace_h, e->trace, sizeof(double) * e->trace_cap, d2h);
-  }
+  // Wait for whatever was queued, also after a failed enqueue.
+  const hipError_t se = hipStreamSynchronize(e->stream);
   if (rc != CE_OK) return rc;
-  CE_HIP(hipStreamSynchronize(e->stream));
+  if (se != hipSuccess) return fail(CE_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
   return CE_OK;
 }
 
@@ -569,6 +585,8 @@ int ce_engine_run_subset(ce_engine* e, const ce_run_config* cfg, const uint64_t*
     return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (e->n_species != 1) return fail(CE_ERR_INVALID_ARGUMENT, "subsets need a single-species engine");
   if (n > e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "more sims than loaded");
+  if (e->loaded() > UINT32_MAX)
+    return fail(CE_ERR_INVALID_ARGUMENT, "subsets index at most 2^32 loaded sims");
   std::vector<uint32_t> idx(n);
   for (size_t i = 0; i < n; ++i) {
     if (sims[i] >= e->loaded()) return fail(CE_ERR_INVALID_ARGUMENT, "sim index out of range");
@@ -602,15 +620,23 @@ int ce_engine_enable_state(ce_engine* e, int32_t enable) {
     return CE_OK;
   }
   if (e->state_x != nullptr) return CE_OK;
-  double *x = nullptr, *v = nullptr;
+  double *x = nullptr, *v = nullptr, *h = nullptr, *e0 = nullptr;
   int rc = hip_alloc(&x, e->total_pos);  // room for T = double
   if (rc == CE_OK) rc = hip_alloc(&v, e->total_pos);
-  if (rc == CE_OK) rc = hip_alloc(&e->state_h, e->capacity);
-  if (rc == CE_OK) rc = hip_alloc(&e->state_e0, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&h, e->capacity);
+  if (rc == CE_OK) rc = hip_alloc(&e0, e->capacity);
+  if (rc != CE_OK) {
+    // all or nothing: a run writes through every state pointer once state_x is set
+    for (double* p : {x, v, h, e0})
+      if (p) (void)hipFree(p);
+    return rc;
+  }
   e->state_x = x;
   e->state_v = v;
+  e->state_h = h;
+  e->state_e0 = e0;
   e->state_prec = -1;
-  return rc;
+  return CE_OK;
 }
 
 int ce_engine_get_state(ce_engine* e, double* pos, double* vel, double* h, double* e0,
@@ -906,6 +932,8 @@ int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const floa
   if (!px || !py || !pz || !pair_const || !inv_mass || !ax || !ay || !az)
     return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (k == 0) return CE_OK;
+  if (k > SIZE_MAX / (8 * sizeof(float) * CE_MAX_ATOMS))
+    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
   const size_t nk = n * k;
   float* d = nullptr;
   const size_t total = 6 * nk + n * n + n;
@@ -947,6 +975,8 @@ int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const f
   if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n out of range");
   if (!px || !py || !pz || !pair_const || !pe) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   if (k == 0) return CE_OK;
+  if (k > SIZE_MAX / (8 * sizeof(float) * CE_MAX_ATOMS))
+    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
   const size_t nk = n * k;
   float* d = nullptr;
   CE_HIP(hipMalloc(&d, (3 * nk + n * n + k) * sizeof(float)));

@@ -41,9 +41,9 @@ enum ce_error {
 /* Termination rule.  CE_STOP_PE_FRACTION is the reference's
  * (driver.hpp:28-35, batched_integrator.hpp:149-169): stop once
  * PE <= pe_stop_fraction * PE0.  CE_STOP_MOMENTUM_OR_TMAX stops when every
- * atom's momentum changed by less than momentum_tol (relative) over the last
- * accepted step, or when the FP64 time reaches t_max (the last step is clamped
- * to land on t_max). */
+ * atom's momentum changed by less than momentum_tol (relative, >= 0) over the
+ * last accepted step, or when the FP64 time reaches t_max (> 0; the last step
+ * is clamped to land on t_max). */
 enum ce_stop_rule { CE_STOP_PE_FRACTION = 0, CE_STOP_MOMENTUM_OR_TMAX = 1 };
 /* Initial step: the reference's Hairer heuristic (integrators.cpp:197-250)
  * or a fixed h0. */

@@ -139,3 +139,30 @@ def test_lane_slot_accounting(gpu):
     assert st["lane_slots"] % 64 == 0
     assert st["attempted_steps"] <= st["lane_slots"]
     assert st["attempted_steps"] > 0.5 * st["lane_slots"]
+
+
+def test_argument_checks_fail_cleanly(gpu):
+    """Sizes that overflow the buffer arithmetic, a negative momentum tolerance
+    and an output the last run did not produce are rejected with
+    invalid_argument; the engine keeps working afterwards, and the rejected
+    download queued no copy (the other output buffers stay untouched)."""
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
+    with pytest.raises(ValueError, match="overflow"):
+        ce.BatchedRK45(mol, 1.0, 2**62, 1e-6, 1e-9)
+    eng = ce.BatchedRK45(mol, 1.0, 500, 1e-6, 1e-9)
+    eng.sample(3, 500, 2.5, 1.0)
+    with pytest.raises(ValueError, match="momentum_tol"):
+        eng.run(ce.RunConfig(**dict(RULE, momentum_tol=-1e-7)))
+    a = eng.run(ce.RunConfig(**RULE))
+    steps = np.full(500, 7, np.uint32)
+    ee = np.zeros(500)
+    P = ce.C.POINTER
+    res = ce._Results(None, None, steps.ctypes.data_as(P(ce.C.c_uint32)), None, None, None,
+                      ee.ctypes.data_as(P(ce.C.c_double)), None, 0, 0)
+    with pytest.raises(ValueError, match="energy_err"):
+        ce._check(ce.library().ce_engine_download(eng._h, ce.C.byref(res)))
+    assert np.all(steps == 7)
+    b = eng.download()
+    np.testing.assert_array_equal(a.momenta, b.momenta)
+    np.testing.assert_array_equal(a.steps, b.steps)
