@@ -16,6 +16,7 @@
 //     momentum_tol, fixed h0, precision) with the reference's defaults.
 #pragma once
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -153,8 +154,9 @@ class BatchedRK45 {
   // Device-side counter-based sampler for sims [0, n): global index first + s.
   void sample(std::uint64_t seed, std::uint64_t first, std::size_t n,
               const UniformSphereSampler::Options& o) {
+    const std::size_t attempts = std::min<std::size_t>(o.max_attempts_per_atom, UINT32_MAX);
     detail::check(ce_engine_sample(h_, seed, first, n, o.radius, o.min_separation,
-                                   static_cast<uint32_t>(o.max_attempts_per_atom)));
+                                   static_cast<uint32_t>(attempts)));
     loaded_ = n;
     dirty_ = false;
   }
