@@ -19,6 +19,7 @@
 #include <iomanip>
 #include <mutex>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -76,24 +77,28 @@ Args parse_args(int argc, char** argv) {
     }
     if (i + 1 >= argc) die("unknown or incomplete argument: " + f);
     const std::string v = argv[++i];
-    if (f == "--sims") a.sims = std::stoull(v);
-    else if (f == "--seed") a.seed = std::stoull(v);
-    else if (f == "--radius") a.radius = std::stod(v);
-    else if (f == "--min-separation") a.min_separation = std::stod(v);
-    else if (f == "--rtol") a.rtol = std::stod(v);
-    else if (f == "--atol") a.atol = std::stod(v);
-    else if (f == "--pe-stop") a.pe_stop = std::stod(v);
-    else if (f == "--stop-rule") a.stop_rule = v;
-    else if (f == "--t-max") a.t_max = std::stod(v);
-    else if (f == "--momentum-tol") a.momentum_tol = std::stod(v);
-    else if (f == "--h0") a.h0 = std::stod(v);
-    else if (f == "--batch") a.batch = std::stoull(v);
-    else if (f == "--sampler") a.sampler = v;
-    else if (f == "--masses") a.masses = parse_list(v);
-    else if (f == "--charges") a.charges = parse_list(v);
-    else if (f == "--out") a.out = v;
-    else if (f == "--progress-every") a.progress_every = std::stoull(v);
-    else die("unknown or incomplete argument: " + f);
+    try {
+      if (f == "--sims") a.sims = std::stoull(v);
+      else if (f == "--seed") a.seed = std::stoull(v);
+      else if (f == "--radius") a.radius = std::stod(v);
+      else if (f == "--min-separation") a.min_separation = std::stod(v);
+      else if (f == "--rtol") a.rtol = std::stod(v);
+      else if (f == "--atol") a.atol = std::stod(v);
+      else if (f == "--pe-stop") a.pe_stop = std::stod(v);
+      else if (f == "--stop-rule") a.stop_rule = v;
+      else if (f == "--t-max") a.t_max = std::stod(v);
+      else if (f == "--momentum-tol") a.momentum_tol = std::stod(v);
+      else if (f == "--h0") a.h0 = std::stod(v);
+      else if (f == "--batch") a.batch = std::stoull(v);
+      else if (f == "--sampler") a.sampler = v;
+      else if (f == "--masses") a.masses = parse_list(v);
+      else if (f == "--charges") a.charges = parse_list(v);
+      else if (f == "--out") a.out = v;
+      else if (f == "--progress-every") a.progress_every = std::stoull(v);
+      else die("unknown or incomplete argument: " + f);
+    } catch (const std::exception&) {  // std::stoull / std::stod on a malformed value
+      die("bad value for " + f + ": " + v);
+    }
   }
   if (a.out.empty()) die("--out is required (no default -- pick your own output location)");
   if (a.stop_rule != "pe" && a.stop_rule != "tmax") die("--stop-rule must be pe or tmax");

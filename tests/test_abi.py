@@ -156,3 +156,19 @@ def test_cpp_mirror_compiles_and_links(tmp_path):
                         "-Wl,-rpath," + os.path.join(ROOT, "coulomb-explosion_amd")],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.parametrize("argv,msg", [
+    (["--sims", "abc", "--out", "x.bin"], "bad value for --sims"),
+    (["--masses", "1,,2", "--out", "x.bin"], "bad value for --masses"),
+    (["--sims", "10"], "--out is required"),
+    (["--stop-rule", "energy", "--out", "x.bin"], "--stop-rule must be"),
+])
+def test_generator_cli_rejects_bad_arguments(tmp_path, argv, msg):
+    """The dataset generator exits with status 1 and a message on malformed or
+    missing flags, before it touches a device."""
+    exe = os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset")
+    if not os.path.exists(exe):
+        pytest.skip("generate_dataset not built")
+    r = subprocess.run([exe] + argv, cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and msg in r.stderr, (r.returncode, r.stderr)
