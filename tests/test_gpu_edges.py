@@ -166,3 +166,32 @@ def test_argument_checks_fail_cleanly(gpu):
     b = eng.download()
     np.testing.assert_array_equal(a.momenta, b.momenta)
     np.testing.assert_array_equal(a.steps, b.steps)
+
+
+def test_forced_failures_end_sims_with_their_status(gpu, oracle):
+    """Failures end a sim with a status, never the run, as the oracle's lane
+    statement has them: max_steps reached (not converged, after exactly
+    max_steps steps), a rejection storm (a fixed h0 of 1e3 a.u. rejected past
+    max_rejections: limit after max_rejections + 1 attempts), and a non-finite
+    state (two coincident atoms: every attempt rejected, then non-finite)."""
+    from oracle import ffi
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in CH4_MASSES])
+    eng = ce.BatchedRK45(mol, 1.0, 64, 1e-6, 1e-9)
+    eng.sample(5, 64, 2.5, 1.0)
+    geom = eng.geometries()
+    coincident = geom.copy()
+    coincident[:, 1] = coincident[:, 0]
+    cases = [("max_steps", dict(max_steps=5), geom, ce.SIM_NOT_CONVERGED, 5, 5),
+             ("storm", dict(h0=1e3, max_rejections=2), geom, ce.SIM_REJECT_LIMIT, 0, 3),
+             ("coincident", {}, coincident, ce.SIM_NONFINITE, 0, 101)]
+    for name, kw, g, status, steps, attempts in cases:
+        eng.set_geometries(g)
+        r = eng.run(ce.RunConfig(**dict(RULE, **kw)))
+        cfg = ffi.make_config(rtol=1e-6, atol=1e-9, **dict(RULE, **kw))
+        o = oracle.lane_run(CH4_MASSES, [1.0] * 5, g, cfg, precision="f32")
+        assert np.all(r.status == status), name
+        assert np.all(r.steps == steps) and np.all(r.attempts == attempts), name
+        assert np.all(r.converged == 0), name
+        np.testing.assert_array_equal(r.status, o.status)
+        np.testing.assert_array_equal(r.attempts, o.attempts)

@@ -1,7 +1,8 @@
 """The headline workload at its full size (10M CH4 sims in one launch), checked
 through properties that do not depend on the batch size: every sim stops by the
 rule, momentum is conserved, the redistributed kinetic energy equals the initial
-potential energy, results do not depend on how the sims are batched (a slice
+potential energy, the kinetic energy splits between the species as the
+equal-momentum estimate says, results do not depend on how the sims are batched (a slice
 re-run in a small engine gives the same bits), and a slice agrees with the
 oracle's per-lane statement."""
 import numpy as np
@@ -41,7 +42,7 @@ def test_every_sim_stops_by_the_rule(full_run):
     assert st["sims"] == N_SIMS and st["attempted_steps"] == int(res.attempts.sum())
 
 
-def test_momentum_conserved_and_energy_redistributed(full_run):
+def test_momentum_conserved_energy_redistributed_and_partitioned(full_run):
     ce, _, res, geom = full_run
     p = res.momenta.reshape(N_SIMS, 5, 3)
     scale = np.linalg.norm(p, axis=-1).sum(axis=1)
@@ -54,6 +55,12 @@ def test_momentum_conserved_and_energy_redistributed(full_run):
             pe0 += 1.0 / np.linalg.norm(geom[:, i] - geom[:, j], axis=-1)
     # redistribution makes KE equal E0 = PE0 (E0 from the f32 potential energy)
     assert np.max(np.abs(ke - pe0) / pe0) < 1e-5
+    # energy partition per species against the equal-|p| prediction n_s/m_s
+    # normalised (demo/sanity_check.py:110-124 checks it to 1%)
+    ke_atom = np.sum(p * p, axis=-1) / (2.0 * m)
+    h_share = ke_atom[:, 1:].sum() / ke_atom.sum()
+    w_c, w_h = 1.0 / m[0], 4.0 / m[1]
+    assert abs(h_share - w_h / (w_c + w_h)) < 0.01
 
 
 def test_slices_rerun_in_small_engines_give_the_same_bits(full_run):
