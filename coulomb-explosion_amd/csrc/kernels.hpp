@@ -719,9 +719,33 @@ __device__ __forceinline__ void flush_totals(const RunArgs& p, Totals& tot) {
   }
 }
 
+// Waves per SIMD the compiler must fit the f32 integrator into (its register
+// budget is 512 / waves): more resident waves hide the issue latency of the
+// step's dependent chains, which one or two waves per SIMD leave exposed (a
+// lone wave issues a VALU instruction every ~4-5 cycles, two or more waves
+// every ~2.5).  The values are the highest occupancy each size reaches without
+// spilling in the step: N = 2 at six waves (80 VGPRs; five with per-sim
+// charges), N = 3 at four, N = 4 and N = 5 at three (CH4: 168 VGPRs, two
+// loop-invariant values spilled and reloaded only outside the step; 8% faster
+// than at two waves).  N = 5 with per-sim charges and N >= 6 would spill the
+// step itself at three waves (measured no faster at N = 5); the compiler's
+// choice stands for them.
+template <class T, int N, int CM>
+constexpr int min_waves_per_simd() {
+  if constexpr (!std::is_same<T, float>::value) {
+    return 1;
+  } else {
+    constexpr int fixed[11] = {1, 1, 6, 4, 3, 3, 1, 1, 1, 1, 1};
+    constexpr int per_sim[11] = {1, 1, 5, 4, 3, 1, 1, 1, 1, 1, 1};
+    return CM == kPerSim ? per_sim[N] : fixed[N];
+  }
+}
+
 // One molecule for the whole launch (segment 0).
 template <class T, int N, int CM>
-__global__ void __launch_bounds__(256) dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
+                               amdgpu_waves_per_eu(min_waves_per_simd<T, N, CM>())))
+dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
   Totals tot;
   run_segment<T, N, CM>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
   flush_totals(p, tot);
