@@ -322,7 +322,8 @@ struct Segment {
 };
 constexpr int kMaxSegments = 4;
 // ctr layout: [0] attempted steps, [1] accepted steps, [2] refill events,
-// [3] lane slots (64 x loop iterations of every wave), [4 + s] work-queue head
+// [3] lane slots (simulation seats x loop iterations of every wave: 64 seats, 32
+// with lane pairs), [4 + s] work-queue head
 // of segment s.
 constexpr int kCtrQueue = 4;
 constexpr int kCtrWords = 8;
@@ -370,6 +371,7 @@ struct RunArgs {
 struct Totals {
   uint32_t att = 0, acc = 0, refills = 0;
   uint32_t iters = 0;  // loop iterations of the wave (the same on every lane)
+  uint64_t slots = 0;  // simulation seats x iterations (64 per iteration, 32 with lane pairs)
 };
 
 // Runs one segment to exhaustion with a persistent per-lane DP5(4): each lane
@@ -572,6 +574,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
     }
     if (__ballot(active) == 0) break;
     ++tot.iters;
+    tot.slots += 64;
 
     // ---- one attempted DP5(4) step on every running lane ----
     if (active && status == kRunning) {
@@ -715,9 +718,15 @@ __device__ __forceinline__ void flush_totals(const RunArgs& p, Totals& tot) {
     atomicAdd(p.ctr + 0, static_cast<unsigned long long>(tot.att));
     atomicAdd(p.ctr + 1, static_cast<unsigned long long>(tot.acc));
     atomicAdd(p.ctr + 2, static_cast<unsigned long long>(tot.refills));
-    atomicAdd(p.ctr + 3, 64ull * tot.iters);
+    atomicAdd(p.ctr + 3, static_cast<unsigned long long>(tot.slots));
   }
 }
+
+}  // namespace ce
+
+#include "kernels_pair.hpp"
+
+namespace ce {
 
 // Waves per SIMD the compiler must fit the f32 integrator into (its register
 // budget is 512 / waves): more resident waves hide the issue latency of the
@@ -737,8 +746,21 @@ constexpr int min_waves_per_simd() {
   } else {
     constexpr int fixed[11] = {1, 1, 6, 4, 3, 3, 1, 1, 1, 1, 1};
     constexpr int per_sim[11] = {1, 1, 5, 4, 3, 1, 1, 1, 1, 1, 1};
+    if constexpr (use_pair_lanes<N, CM>()) return 2;
     return CM == kPerSim ? per_sim[N] : fixed[N];
   }
+}
+
+// The integrator of one segment: the pair-lane form where it applies (f32,
+// N = 8 with per-sim or uniform pair constants), else one lane per sim.
+template <class T, int N, int CM>
+__device__ __forceinline__ void run_any(const RunArgs& p, const Segment& sg,
+                                        const MolConst<T, N>& mc, unsigned long long* queue,
+                                        Totals& tot) {
+  if constexpr (std::is_same<T, float>::value && use_pair_lanes<N, CM>())
+    run_segment_pair<N, CM>(p, sg, mc, queue, tot);
+  else
+    run_segment<T, N, CM>(p, sg, mc, queue, tot);
 }
 
 // One molecule for the whole launch (segment 0).
@@ -747,7 +769,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
                                amdgpu_waves_per_eu(min_waves_per_simd<T, N, CM>())))
 dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
   Totals tot;
-  run_segment<T, N, CM>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
+  run_any<T, N, CM>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
   flush_totals(p, tot);
 }
 
@@ -767,10 +789,10 @@ template <class T, int N0, int N1, int N2, int N3, int CM>
 __global__ void __launch_bounds__(256) dp5_mixed(const RunArgs p,
                                                  const MixedConst<T, N0, N1, N2, N3> mc) {
   Totals tot;
-  if (p.n_segments > 0) run_segment<T, N0, CM>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
-  if (p.n_segments > 1) run_segment<T, N1, CM>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);
-  if (p.n_segments > 2) run_segment<T, N2, CM>(p, p.seg[2], mc.m2, p.ctr + kCtrQueue + 2, tot);
-  if (p.n_segments > 3) run_segment<T, N3, CM>(p, p.seg[3], mc.m3, p.ctr + kCtrQueue + 3, tot);
+  if (p.n_segments > 0) run_any<T, N0, CM>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
+  if (p.n_segments > 1) run_any<T, N1, CM>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);
+  if (p.n_segments > 2) run_any<T, N2, CM>(p, p.seg[2], mc.m2, p.ctr + kCtrQueue + 2, tot);
+  if (p.n_segments > 3) run_any<T, N3, CM>(p, p.seg[3], mc.m3, p.ctr + kCtrQueue + 3, tot);
   flush_totals(p, tot);
 }
 

@@ -127,9 +127,11 @@ typedef struct {
   uint64_t attempted_steps;  /* summed over sims */
   uint64_t accepted_steps;
   uint64_t refills;          /* wave refill events */
-  uint64_t lane_slots;       /* 64 x integration-loop iterations of every wave: the
-                                share attempted_steps / lane_slots of lane slots did
-                                a step (the rest idled: refill wait, tails) */
+  uint64_t lane_slots;       /* simulation seats x integration-loop iterations of
+                                every wave (64 seats per wave; 32 where a simulation
+                                spans a pair of lanes, f32 N = 8): the share
+                                attempted_steps / lane_slots of seats did a step
+                                (the rest idled: refill wait, tails) */
   int32_t grid_blocks;
   int32_t block_threads;
   int32_t n_cus;

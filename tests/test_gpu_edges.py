@@ -127,9 +127,10 @@ def test_energy_diagnostic_is_opt_in(gpu):
 
 
 def test_lane_slot_accounting(gpu):
-    """stats()["lane_slots"] is 64 x the waves' loop iterations: a multiple of 64,
-    at least the attempted steps (one attempt per running lane per iteration),
-    and for a large batch mostly used (refill keeps the lanes busy)."""
+    """stats()["lane_slots"] is 64 x the waves' loop iterations (32 where a sim
+    spans two lanes): a multiple of 64 (32), at least the attempted steps (one
+    attempt per running sim per iteration), and for a large batch mostly used
+    (refill keeps the lanes busy)."""
     ce = gpu
     mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
     eng = ce.BatchedRK45(mol, 1.0, 1000000, 1e-6, 1e-9)
@@ -139,6 +140,16 @@ def test_lane_slot_accounting(gpu):
     assert st["lane_slots"] % 64 == 0
     assert st["attempted_steps"] <= st["lane_slots"]
     assert st["attempted_steps"] > 0.5 * st["lane_slots"]
+    # C2H6 (N = 8): one sim per lane pair, 32 seats per wave iteration
+    mol8 = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] * 2 +
+                       [ce.Atom("H", CH4_MASSES[1], 1.0)] * 6)
+    eng8 = ce.BatchedRK45(mol8, 1.0, 200000, 1e-6, 1e-9)
+    eng8.sample(8, 200000, 3.5, 0.9)
+    eng8.run(ce.RunConfig(**RULE), download=False)
+    st8 = eng8.stats()
+    assert st8["lane_slots"] % 32 == 0
+    assert st8["attempted_steps"] <= st8["lane_slots"]
+    assert st8["attempted_steps"] > 0.5 * st8["lane_slots"]
 
 
 def test_argument_checks_fail_cleanly(gpu):
