@@ -9,9 +9,10 @@
 // form, at one wave per SIMD, issues VALU only every ~4.5 cycles.  The pair
 // forces are split between the two lanes: each lane evaluates its own atoms'
 // pairs, the cross pairs at atom-index offsets 1..(NA-1)/2 (the partner lane
-// covers the opposite offsets) with the reaction on the partner's atoms sent
-// back through one DPP exchange, and the offset-0 (and, for even NA, offset
-// NA/2) cross pairs for its own atoms only.  Every other quantity of the step
+// covers the opposite offsets) and, for even NA, the offset-NA/2 pairs of its
+// first NA/2 atoms (the partner lane's first NA/2 are the rest), with the
+// reaction on the partner's atoms sent back through one DPP exchange, and the
+// offset-0 cross pairs for its own atoms only (both lanes evaluate those).  Every other quantity of the step
 // is per component (the lane's own 3·NA) except the error norm, the stop test
 // and the energies, which combine the two halves as a + partner(a) —
 // commutative, so both lanes hold the same bits and take the same decisions.
@@ -150,21 +151,33 @@ __device__ __forceinline__ void pair_force(const float (&xo)[3 * NA], float (&a)
       }
     });
   });
-  // cross pairs both lanes evaluate (offset 0, and NA/2 for even NA): own side
-  // only, half of the pair's energy each
-  constexpr int n_own_only = (NA % 2 == 0) ? 2 : 1;
-  static_for<0, n_own_only>([&](auto K) {
-    constexpr int off = decltype(K)::value == 0 ? 0 : NA / 2;
-    static_for<0, NA>([&](auto I) {
+  // even NA, offset NA/2: own atoms i < NA/2 both-sided (the partner lane's
+  // i < NA/2 are exactly the other half of these pairs)
+  if constexpr (NA % 2 == 0) {
+    static_for<0, NA / 2>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      constexpr int j = (i + off) % NA;
+      constexpr int j = i + NA / 2;
       const float c = pk.template cross<i, j>();
       float r[3], s;
       const float inv = pair_term(&xo[3 * i], &xp[3 * j], c, r, s);
-      if constexpr (WITH_PE) u = fmaT(0.5f * c, inv, u);
+      if constexpr (WITH_PE) u = fmaT(c, inv, u);
 #pragma unroll
-      for (int d = 0; d < 3; ++d) a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
+      for (int d = 0; d < 3; ++d) {
+        a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
+        ap[3 * j + d] = fmaT(-r[d], s, ap[3 * j + d]);
+      }
     });
+  }
+  // offset 0 (atom i with the partner's atom i): both lanes evaluate these, each
+  // for its own atom only, with half of the pair's energy
+  static_for<0, NA>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    const float c = pk.template cross<i, i>();
+    float r[3], s;
+    const float inv = pair_term(&xo[3 * i], &xp[3 * i], c, r, s);
+    if constexpr (WITH_PE) u = fmaT(0.5f * c, inv, u);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
   });
 #pragma unroll
   for (int q = 0; q < NA3; ++q) a[q] += xchg(ap[q]);
