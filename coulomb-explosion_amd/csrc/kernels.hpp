@@ -382,7 +382,7 @@ struct Totals {
 // one attempted step for every running lane, so the wave stays convergent
 // except for the short init / finalize branches.  Returns when the queue is
 // exhausted and every lane of the wave has finished.
-template <class T, int N, int CM>
+template <class T, int N, int CM, bool DEFER_K1 = false>
 __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
                                             const MolConst<T, N>& mc,
                                             unsigned long long* queue, Totals& tot) {
@@ -401,6 +401,9 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
   uint32_t steps = 0, att = 0;
   int nrej = 0;
   bool rej_once = false;
+  // With DEFER_K1 a fresh sim's k1 comes from its first step (see the init).
+  constexpr bool kDeferK1 = DEFER_K1;
+  bool k1_pending = false;  // k1 of a fresh sim still to be taken from its first step
   uint64_t loc = 0;  // index of the lane's sim inside the segment
   bool active = false;
   int status = kRunning;
@@ -509,6 +512,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         att = 0;
         nrej = 0;
         rej_once = false;
+        k1_pending = false;
         t = 0.0;
         status = kRunning;
         if (p.resume) {
@@ -550,10 +554,18 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           status = kSimSamplerFail;
           e0 = T(0);
         } else {
-          T pe0;
           // The stages hold forces F = m a; 1/m enters via per-atom step factors.
-          lane_force<T, N, CM, true, false>(x, k[0], pe0, pk);
-          e0 = pe0;
+          if (kDeferK1 && p.h0_mode == kH0Fixed) {
+            // The sim starts at rest, so the first step's second stage sits at x0
+            // exactly (x0 + h c2 * 0) and its force is F(x0) to the bit: k1 is
+            // taken from there, and only the potential energy is needed here.
+            e0 = lane_pe<T, N, CM>(x, pk);
+            k1_pending = true;
+          } else {
+            T pe0;
+            lane_force<T, N, CM, true, false>(x, k[0], pe0, pk);
+            e0 = pe0;
+          }
           if (p.stop_rule == kStopPE &&
               static_cast<double>(e0) <= p.pe_stop_fraction * static_cast<double>(e0)) {
             status = kSimOkPE;
@@ -610,6 +622,18 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
         T unused;
         lane_force<T, N, CM, false, false>(xs, k[s], unused, pk);
+        if constexpr (kDeferK1 && s == 1) {
+          // fresh sims: stage 2 was at x0, so F(x0) is the k1 their init skipped
+          // (k1 is first read by stage 3)
+          static_assert(!has_terms<CoefAA<1>, 1>(), "stage 2 must not read k1");
+          if (__ballot(k1_pending) != 0) {
+            if (k1_pending) {
+#pragma unroll
+              for (int q = 0; q < N3; ++q) k[0][q] = k[1][q];
+            }
+            k1_pending = false;
+          }
+        }
       });
       // Stage 7 (FSAL): the 5th-order solution; F(x_new) is the next k1.  The
       // embedded error in the RMS norm (scale = atol + rtol max(|y_old|, |y_new|),
@@ -753,15 +777,23 @@ constexpr int min_waves_per_simd() {
 
 // The integrator of one segment: the pair-lane form where it applies (f32,
 // N = 8 with per-sim or uniform pair constants), else one lane per sim.
-template <class T, int N, int CM>
+template <class T, int N, int CM, bool DEFER_K1 = false>
 __device__ __forceinline__ void run_any(const RunArgs& p, const Segment& sg,
                                         const MolConst<T, N>& mc, unsigned long long* queue,
                                         Totals& tot) {
   if constexpr (std::is_same<T, float>::value && use_pair_lanes<N, CM>())
     run_segment_pair<N, CM>(p, sg, mc, queue, tot);
   else
-    run_segment<T, N, CM>(p, sg, mc, queue, tot);
+    run_segment<T, N, CM, DEFER_K1>(p, sg, mc, queue, tot);
 }
+
+// Whether a single-size launch takes a fresh sim's k1 from its first step: for
+// N = 4 and 5, where the skipped initial force outweighs the per-step check
+// (CH4: 14.1 -> 13.7 ms).  Smaller molecules' forces are too cheap to pay for
+// it, the larger ones lose occupancy to the flag, and in the mixed launch it
+// measured slower.
+template <int N>
+constexpr bool defer_k1() { return N == 4 || N == 5; }
 
 // One molecule for the whole launch (segment 0).
 template <class T, int N, int CM>
@@ -769,7 +801,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
                                amdgpu_waves_per_eu(min_waves_per_simd<T, N, CM>())))
 dp5_persistent(const RunArgs p, const MolConst<T, N> mc) {
   Totals tot;
-  run_any<T, N, CM>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
+  run_any<T, N, CM, defer_k1<N>()>(p, p.seg[0], mc, p.ctr + kCtrQueue + p.queue_slot, tot);
   flush_totals(p, tot);
 }
 
