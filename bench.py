@@ -57,8 +57,8 @@ SEED = 49188291
 PEAK_FP32_TFLOPS = 157.3   # MI355X published FP32 peak
 # HBM bytes per sim of one integrator launch, from the FETCH_SIZE / WRITE_SIZE passes
 # scaled by the plain-copy calibration (x2.0 read, x1.0 write), 2M-sim CH4 launch:
-# 125.3 B read + 228.0 B written (profiles/dp5_persistent_ch4_counters.md).
-MEASURED_HBM_BYTES_PER_SIM = {"ch4": 125.3 + 228.0}
+# 125.0 B read + 232.6 B written (profiles/dp5_persistent_ch4_counters.md).
+MEASURED_HBM_BYTES_PER_SIM = {"ch4": 125.0 + 232.6}
 MIXED = [([N14, N14], 2), ([C_, O_, O_], 3), (CH4_MASSES, 5), ([C_, C_] + [H_] * 6, 8)]
 N10_MASSES = [C_, C_, O_, O_] + [H_] * 6
 
