@@ -100,6 +100,11 @@ class _LaunchConfig(C.Structure):
 MIXED_AUTO, MIXED_ONE_LAUNCH, MIXED_PER_SPECIES = 0, 1, 2
 
 
+class _DeviceResults(C.Structure):
+    _fields_ = [("momenta", C.c_void_p), ("momenta_f32", C.c_void_p), ("steps", C.c_void_p),
+                ("attempts", C.c_void_p), ("status", C.c_void_p), ("t_final", C.c_void_p)]
+
+
 def _load_library(path: str = LIB_PATH) -> C.CDLL:
     if not os.path.exists(path):
         raise ImportError(
@@ -159,6 +164,7 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_closure_features": ([P, P, P, P, S], C.c_int),
         "ce_closure_run": ([P, P, P, C.c_double, C.c_double, P], C.c_int),
         "ce_closure_get_stats": ([P, P], C.c_int),
+        "ce_run_device_noalloc": ([P, P, P, P, S, P, P], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -187,6 +193,25 @@ def device_count() -> int:
     n = C.c_int(0)
     _lib.ce_device_count(C.byref(n))
     return n.value
+
+
+def run_device_noalloc(mol: "Molecule", coulomb_k: float, rtol: float, atol: float,
+                       cfg: "RunConfig", geom_ptr: int, n_sims: int, out: dict,
+                       stream: int = 0) -> None:
+    """ce_run_device_noalloc: integrate n_sims sims whose geometry (AoS float64
+    [sim][n][3]) is at device address `geom_ptr` into caller-owned device buffers
+    `out` (addresses under the keys momenta, momenta_f32, steps, attempts, status,
+    t_final), queued on the HIP stream `stream` and not waited for.  No device
+    memory is allocated."""
+    m = np.ascontiguousarray(mol.masses)
+    q = np.ascontiguousarray(mol.charges)
+    desc = _MoleculeDesc(mol.size(), 0, m.ctypes.data_as(C.POINTER(C.c_double)),
+                         q.ctypes.data_as(C.POINTER(C.c_double)), float(coulomb_k))
+    tol = _Tolerances(float(rtol), float(atol))
+    res = _DeviceResults(*[out[k] for k, _ in _DeviceResults._fields_])
+    c = cfg._c()
+    _check(_lib.ce_run_device_noalloc(C.byref(desc), C.byref(tol), C.byref(c), geom_ptr,
+                                      int(n_sims), C.byref(res), stream))
 
 
 # --------------------------------------------------------- reference mirrors ----

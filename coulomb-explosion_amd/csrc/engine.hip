@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -673,9 +674,29 @@ int ce_engine_get_state(ce_engine* e, double* pos, double* vel, double* h, doubl
 
 namespace ce {
 
-// Runs the loaded sims (or the segment-local sims of the device list `list`,
-// single-species engines) with `cfg`; results stay on the device.
-// Launch request of a run over the loaded sims (or the device list).
+// The run configuration, tolerances and launch tunables of a run, shared by the
+// engine's runs and ce_run_device_noalloc.
+void fill_run_args(ce::RunArgs* a, const ce_run_config* cfg, double rtol, double atol,
+                   double coulomb_k, const ce_launch_config& lc) {
+  a->stop_rule = cfg->stop_rule;
+  a->rtol = rtol;
+  a->atol = atol;
+  a->max_dt = cfg->max_dt;
+  a->t_max = cfg->t_max;
+  a->pe_stop_fraction = cfg->pe_stop_fraction;
+  a->momentum_tol = cfg->momentum_tol;
+  a->h0 = cfg->h0;
+  a->coulomb_k = coulomb_k;
+  a->max_steps = cfg->max_steps;
+  a->redistribute = cfg->redistribute_energy ? 1 : 0;
+  a->h0_mode = cfg->h0_mode;
+  a->max_rej = cfg->max_rejections;
+  a->refill_threshold = std::max(1, std::min(64, lc.refill_threshold));
+  a->chunk = std::max(64, lc.chunk);
+}
+
+// Launch request of a run over the loaded sims (or, single-species engines, the
+// segment-local sims of the device list `list`).
 int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, size_t n_list,
                 ce::LaunchReq* req) {
   int rc = check_config(cfg);
@@ -686,6 +707,7 @@ int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, si
     return fail(CE_ERR_INVALID_ARGUMENT, "sim lists need a single-species engine");
   *req = ce::LaunchReq{};
   ce::RunArgs& a = req->args;
+  fill_run_args(&a, cfg, e->rtol, e->atol, e->coulomb_k, e->launch);
   a.geom = e->geom;
   a.geom_status = e->sampled ? e->gstat : nullptr;
   a.qsim = e->have_charges ? e->qsim : nullptr;
@@ -699,21 +721,6 @@ int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, si
   a.eerr = e->energy_diag ? e->eerr : nullptr;
   a.trace_h = e->trace;
   a.trace_cap = e->trace_cap;
-  a.stop_rule = cfg->stop_rule;
-  a.rtol = e->rtol;
-  a.atol = e->atol;
-  a.max_dt = cfg->max_dt;
-  a.t_max = cfg->t_max;
-  a.pe_stop_fraction = cfg->pe_stop_fraction;
-  a.momentum_tol = cfg->momentum_tol;
-  a.h0 = cfg->h0;
-  a.coulomb_k = e->coulomb_k;
-  a.max_steps = cfg->max_steps;
-  a.redistribute = cfg->redistribute_energy ? 1 : 0;
-  a.h0_mode = cfg->h0_mode;
-  a.max_rej = cfg->max_rejections;
-  a.refill_threshold = std::max(1, std::min(64, e->launch.refill_threshold));
-  a.chunk = std::max(64, e->launch.chunk);
   a.n_segments = e->n_species;
   a.queue_slot = 0;
   a.state_x = e->state_x;
@@ -1007,3 +1014,110 @@ int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const f
 }
 
 }  // extern "C"
+
+// ---- ce_run_device_noalloc: a run with no device allocation ------------------
+namespace {
+
+// The integrator's scratch memory for ce_run_device_noalloc: the work-queue
+// head and the run counters (ce::kCtrWords 64-bit words, 64 B).  Every GPU holds
+// its own copy at its own address until the process ends.  In host code this
+// name is a host-side stand-in, not a device address: kernels and copies take
+// the address hipGetSymbolAddress returns for the current device
+// (noalloc_device), never `scratch` itself.
+constexpr int kScratchWords = 8;
+static_assert(kScratchWords >= ce::kCtrWords, "scratch must hold the run counters");
+static __device__ __align__(256) unsigned long long scratch[kScratchWords];
+
+struct NoallocDevice {
+  unsigned long long* scratch = nullptr;  // device address of `scratch` on this GPU
+  int n_cus = 0;
+};
+constexpr int kNoallocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+NoallocDevice g_noalloc[kNoallocMaxDevices];
+
+// Set-up of the current device, once: the device address of its `scratch` and
+// its compute-unit count (the persistent kernels launch per CU).
+int noalloc_device(NoallocDevice* out) {
+  int dev = 0;
+  CE_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoallocMaxDevices)
+    return fail(CE_ERR_RUNTIME, "ce_run_device_noalloc: device id beyond 63");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  NoallocDevice& d = g_noalloc[dev];
+  if (d.scratch == nullptr) {
+    int cus = 0;
+    CE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    void* p = nullptr;
+    CE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (p == nullptr) return fail(CE_ERR_HIP, "hipGetSymbolAddress(scratch) gave NULL");
+    d.n_cus = cus;
+    d.scratch = static_cast<unsigned long long*>(p);
+  }
+  *out = d;
+  return CE_OK;
+}
+
+}  // namespace
+
+extern "C" int ce_run_device_noalloc(const ce_molecule_desc* mol, const ce_tolerances* tol,
+                                     const ce_run_config* cfg, const double* geom,
+                                     size_t n_sims, const ce_device_results* out,
+                                     void* stream) {
+  if (mol == nullptr || tol == nullptr || out == nullptr)
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (mol->n_atoms < 1 || mol->n_atoms > CE_MAX_ATOMS)
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms must be in [1, CE_MAX_ATOMS]");
+  if (mol->masses == nullptr || mol->charges == nullptr)
+    return fail(CE_ERR_INVALID_ARGUMENT, "masses/charges NULL");
+  int rc = check_species(mol->masses, mol->n_atoms);
+  if (rc != CE_OK) return rc;
+  if (!(tol->rtol > 0.0) || !(tol->atol >= 0.0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "rtol must be > 0 and atol >= 0");
+  rc = check_config(cfg);
+  if (rc != CE_OK) return rc;
+  if (cfg->resume) return fail(CE_ERR_INVALID_ARGUMENT, "ce_run_device_noalloc keeps no state: no resume");
+  if (n_sims > 0 && (geom == nullptr || out->momenta == nullptr || out->momenta_f32 == nullptr ||
+                     out->steps == nullptr || out->attempts == nullptr ||
+                     out->status == nullptr || out->t_final == nullptr))
+    return fail(CE_ERR_INVALID_ARGUMENT, "geometry and every result pointer are required");
+  if (n_sims > SIZE_MAX / (3 * CE_MAX_ATOMS * sizeof(double)))
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_sims too large (buffer sizes overflow)");
+  if (n_sims == 0) return CE_OK;
+  NoallocDevice dev;
+  rc = noalloc_device(&dev);
+  if (rc != CE_OK) return rc;
+
+  ce::LaunchReq req{};
+  ce::RunArgs& a = req.args;
+  const ce_launch_config lc = {0, 0, 4, 64, CE_MIXED_AUTO};  // the engine's defaults
+  ce::fill_run_args(&a, cfg, tol->rtol, tol->atol, mol->coulomb_k, lc);
+  a.geom = geom;
+  a.ctr = dev.scratch;
+  a.mom = out->momenta;
+  a.mom32 = out->momenta_f32;
+  a.steps = out->steps;
+  a.attempts = out->attempts;
+  a.status = out->status;
+  a.t_final = out->t_final;
+  a.n_segments = 1;
+  a.seg[0] = ce::Segment{0, n_sims, 0, 0, mol->n_atoms, 0};
+  req.species[0].n = mol->n_atoms;
+  for (int i = 0; i < mol->n_atoms; ++i) {
+    req.species[0].masses[i] = mol->masses[i];
+    req.species[0].charges[i] = mol->charges[i];
+  }
+  req.coulomb_k = mol->coulomb_k;
+  req.n_cus = dev.n_cus;
+  req.stream = static_cast<hipStream_t>(stream);
+  // The counters start at zero on every call (a kernel, not a memset node).
+  hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, req.stream, dev.scratch, ce::kCtrWords);
+  CE_HIP(hipGetLastError());
+  ce::LaunchInfo info;
+  bool bad_n = false;
+  const hipError_t he = (cfg->precision == CE_PRECISION_F64) ? ce::launch_dp5_f64(req, &info, &bad_n)
+                                                             : ce::launch_dp5_f32(req, &info, &bad_n);
+  if (bad_n) return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  if (he != hipSuccess) return fail(CE_ERR_HIP, std::string("dp5 launch: ") + hipGetErrorString(he));
+  return CE_OK;
+}

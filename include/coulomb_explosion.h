@@ -262,6 +262,34 @@ int ce_engine_get_state(ce_engine* eng, double* pos, double* vel, double* h, dou
 int ce_engine_run_subset(ce_engine* eng, const ce_run_config* cfg, const uint64_t* sims,
                          size_t n, ce_results* out);
 
+/* Results of ce_run_device_noalloc: DEVICE memory the caller allocated, laid
+ * out as in ce_results (per-atom arrays [(sim*n + i)*3 + c], per-sim arrays of
+ * n_sims entries).  Every field is required. */
+typedef struct {
+  double* momenta;
+  float* momenta_f32;
+  uint32_t* steps;
+  uint32_t* attempts;
+  uint8_t* status;
+  double* t_final;
+} ce_device_results;
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the integration of ce_engine_create + ce_engine_set_geometries
+ * + ce_engine_run (same kernels, same bits per sim) over caller-owned device
+ * buffers, queued on `stream` (a hipStream_t; NULL = the null stream) and not
+ * waited for.  geom is DEVICE memory, AoS doubles [(sim*n + i)*3 + c]; fixed
+ * charges from `mol`; no trace, energy diagnostic or saved state; cfg->resume
+ * must be 0.  It calls no hipMalloc / hipFree: the kernels' only scratch memory
+ * (the work-queue head and run counters, 64 bytes, whatever n_sims is) is a
+ * static device array that every GPU holds until the process ends, looked up
+ * once per device on the first call there.  Calls that overlap on one GPU share
+ * that array and race: serialize them (one stream, or wait between calls). */
+int ce_run_device_noalloc(const ce_molecule_desc* mol, const ce_tolerances* tol,
+                          const ce_run_config* cfg, const double* geom, size_t n_sims,
+                          const ce_device_results* out, void* stream);
+
 /* Dataset records of the reference generator (examples/generate_dataset.cpp:
  * 198-217; numpy dtype examples/to_parquet.py:34-42): per sim, packed
  * little-endian, no padding -- n*3 f32 initial positions, n*3 f32 final momenta

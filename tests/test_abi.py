@@ -38,7 +38,7 @@ def test_library_is_built_for_gfx950(ce):
 STRUCTS = {"ce_molecule_desc": "_MoleculeDesc", "ce_tolerances": "_Tolerances",
            "ce_run_config": "_RunConfig", "ce_results": "_Results", "ce_run_stats": "_RunStats",
            "ce_species_desc": "_SpeciesDesc", "ce_closure_stats": "_ClosureStats",
-           "ce_launch_config": "_LaunchConfig"}
+           "ce_launch_config": "_LaunchConfig", "ce_device_results": "_DeviceResults"}
 
 
 def test_struct_layouts_match_header(ce, tmp_path):
@@ -109,6 +109,7 @@ def test_null_arguments_are_rejected(ce):
     assert lib.ce_engine_set_geometries(None, None, 0) == ce.CE_ERR_INVALID_ARGUMENT
     assert lib.ce_host_sampler_sample(None, 5, 1, None) == ce.CE_ERR_INVALID_ARGUMENT
     assert lib.ce_batched_accelerations_f32(0, 1, *([None] * 8)) == ce.CE_ERR_INVALID_ARGUMENT
+    assert lib.ce_run_device_noalloc(*([None] * 4), 1, None, None) == ce.CE_ERR_INVALID_ARGUMENT
 
 
 def test_host_sampler_reproduces_reference_stream(ce):
