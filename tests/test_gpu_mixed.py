@@ -45,6 +45,30 @@ def test_species_sampler_bit_exact(gpu, oracle):
         start += s.count
 
 
+def test_species_sampler_binding_cap(gpu, oracle):
+    """A cap on the total charge that binds (levels {1,2,3}, N = 5, total <= 8):
+    the device sampler's redraws give the oracle's charges bit for bit, and the
+    integration with those charges agrees with the oracle's per-lane form
+    (f64: same steps, <= 1e-12)."""
+    ce = gpu
+    from oracle import ffi
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in [C, H, H, H, H]])
+    sp = [ce.Species(mol, 3000, 2.5, 1.0, charge_levels=3, max_total_charge=8)]
+    mb = ce.MixedBatch(sp, 1.0, 1e-10, 1e-14)
+    mb.sample(seed=404, first_index=77)
+    g, q = mb.geometries()[0], mb.charges()[0]
+    ref_g, ref_q, st, fails = oracle.device_sample_q(404, 77, 3000, 5, 2.5, 1.0, 3, 8)
+    assert fails == 0
+    np.testing.assert_array_equal(g, ref_g)
+    np.testing.assert_array_equal(q, ref_q)
+    assert q.sum(axis=1).max() == 8 and (q == 3).any()
+    res = mb.run(ce.RunConfig(precision=1, **RULE))[0]
+    ref = oracle.lane_run_q(mol.masses, q, g, ffi.make_config(rtol=1e-10, atol=1e-14, **RULE),
+                            precision="f64")
+    np.testing.assert_array_equal(res["steps"], ref.steps)
+    assert per_atom_rel(res["momenta"], ref.momenta).max() < 1e-12
+
+
 @pytest.mark.parametrize("precision,tol", [(0, 1e-5), (1, 1e-12)])
 def test_mixed_launch_vs_oracle(gpu, oracle, precision, tol):
     ce = gpu
