@@ -673,6 +673,11 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       const T en2 = (exs + evs) * inv6n;
       // 0.9 err^(-1/5) with err = sqrt(en2): 0.9 * 2^(-0.1 log2 en2).
       const T grow = T(0.9) * exp2T(T(-0.1) * log2T(en2));
+      // The step that lands on t_max (shortened to the remaining time, or ending
+      // there in FP64) ends the sim by the time limit: the momentum test measures
+      // convergence only over steps the controller chose, so it is not applied
+      // to that one.
+      const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
       if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
@@ -686,7 +691,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
                                fmaT(vn[3 * i + 1], vn[3 * i + 1], vn[3 * i] * vn[3 * i]));
             return num < tol2 * den;
           };
-          settled = atom_settled(0);
+          settled = !final_step && atom_settled(0);
           if (__ballot(settled) != 0) {
 #pragma unroll
             for (int i = 1; i < N; ++i) settled = settled && atom_settled(i);
@@ -700,7 +705,6 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
         T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
-        const bool final_step = by_time && hh >= cap;
         // After the step that lands on t_max the proposal is kept unclamped: the
         // sim is finished unless it is resumed from its saved state.
         h = minT(hh * f, final_step ? max_dt : cap);

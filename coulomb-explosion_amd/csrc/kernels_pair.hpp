@@ -522,6 +522,9 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
       }
       const T en2 = (pair_sum(exs) + pair_sum(evs)) * inv6n;
       const T grow = T(0.9) * exp2T(T(-0.1) * log2T(en2));
+      // The step that lands on t_max ends the sim by the time limit; the momentum
+      // test is not applied to it (its change covers a shortened interval).
+      const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
       if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
@@ -534,7 +537,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
             return num < tol2 * den;
           };
           // own atom 0 first, the rest only where some pair of the wave passes it
-          int ok = atom_settled(0) ? 1 : 0;
+          int ok = (!final_step && atom_settled(0)) ? 1 : 0;
           ok = ok & xchg(ok);
           if (__ballot(ok != 0) != 0) {
 #pragma unroll
@@ -551,7 +554,6 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         }
         T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
-        const bool final_step = by_time && hh >= cap;
         h = minT(hh * f, final_step ? max_dt : cap);
         rej_once = false;
         nrej = 0;

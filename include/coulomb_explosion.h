@@ -43,7 +43,8 @@ enum ce_error {
  * PE <= pe_stop_fraction * PE0.  CE_STOP_MOMENTUM_OR_TMAX stops when every
  * atom's momentum changed by less than momentum_tol (relative, >= 0) over the
  * last accepted step, or when the FP64 time reaches t_max (> 0; the last step
- * is clamped to land on t_max). */
+ * is clamped to land on t_max, and the momentum test is not applied to that
+ * shortened step: the sim ends with CE_SIM_OK_TMAX). */
 enum ce_stop_rule { CE_STOP_PE_FRACTION = 0, CE_STOP_MOMENTUM_OR_TMAX = 1 };
 /* Initial step: the reference's Hairer heuristic (integrators.cpp:197-250)
  * or a fixed h0. */

@@ -142,11 +142,11 @@ class Lockstep {
           }
         }
       } else {
-        final_step = final_step && h >= cap;
+        final_step = final_step && (h >= cap || t + static_cast<double>(h) >= cfg->t_max);
         t = final_step ? cfg->t_max : t + static_cast<double>(h);
         for (int l = 0; l < K; ++l) {
           if (done[l]) continue;
-          bool settled = true;
+          bool settled = !final_step;  // not tested on the step landing on t_max
           for (int i = 0; i < n_ && settled; ++i) {
             const int o = i * K + l;
             const T dx = vel_.x[o] - vold_.x[o], dy = vel_.y[o] - vold_.y[o],

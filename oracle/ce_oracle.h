@@ -134,6 +134,14 @@ int ceo_lane_run_q_f32(const ceo_molecule* mol, const ceo_config* cfg, const dou
                        const double* charges, size_t n_sims, ceo_results* out);
 int ceo_lane_run_q_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                        const double* charges, size_t n_sims, ceo_results* out);
+/* The per-lane form stopped at t_cut and continued (momentum-or-t_max rule):
+ * a sim that reaches t_cut by the time limit continues to cfg->t_max from its
+ * state there, with the step size proposed after the step that landed on
+ * t_cut -- the engine's saved-state resume.  charges may be NULL (the
+ * molecule's); precision 0 = f32, 1 = f64.  -1 for t_cut <= 0 or another rule. */
+int ceo_lane_run_cut(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                     const double* charges, size_t n_sims, double t_cut, int32_t precision,
+                     ceo_results* out);
 /* The derived coefficient tables of the per-lane form, in double:
  * c[7], aa[7*7] (A*A), b[7], e[7], ea[7] (E*A).  Written for inspection. */
 void ceo_lane_coefficients(double* c, double* aa, double* b, double* e, double* ea);

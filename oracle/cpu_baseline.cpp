@@ -152,11 +152,11 @@ class Batch {
             ++n_done;
           }
       } else {
-        final_step = final_step && h >= cap;
+        final_step = final_step && (h >= cap || t + static_cast<double>(h) >= cfg->t_max);
         t = final_step ? cfg->t_max : t + static_cast<double>(h);
         for (int l = 0; l < K; ++l) {
           if (done_[l]) continue;
-          bool settled = true;
+          bool settled = !final_step;  // not tested on the step landing on t_max
           for (int i = 0; i < n_ && settled; ++i) {
             float num = 0.0f, den = 0.0f;
             for (int c = 0; c < 3; ++c) {

@@ -257,6 +257,20 @@ class Oracle:
             return fn(mol, cfg_, pos_, q.ctypes.data, ns, res)
         return self._run(call, masses, np.ones(len(masses)), pos, cfg, k, trace_cap=trace_cap)
 
+    def lane_run_cut(self, masses, charges, pos, cfg, t_cut, k=1.0, precision="f32",
+                     charges_per_sim=None, trace_cap=0):
+        """Per-lane form stopped at t_cut and continued from there to cfg.t_max
+        (the engine's cut + resume)."""
+        fn = self.lib.ceo_lane_run_cut
+        fn.argtypes = [C.c_void_p] * 4 + [C.c_size_t, C.c_double, C.c_int32, C.c_void_p]
+        q = (None if charges_per_sim is None
+             else np.ascontiguousarray(charges_per_sim, dtype=np.float64))
+
+        def call(mol, cfg_, pos_, ns, res):
+            return fn(mol, cfg_, pos_, None if q is None else q.ctypes.data, ns, t_cut,
+                      1 if precision == "f64" else 0, res)
+        return self._run(call, masses, charges, pos, cfg, k, trace_cap=trace_cap)
+
     def batched_run(self, masses, charges, pos, cfg, lanes=8, k=1.0, precision="f32"):
         fn = self.lib.ceo_batched_run_f32 if precision == "f32" else self.lib.ceo_batched_run_f64
         return self._run(fn, masses, charges, pos, cfg, k, lanes=lanes)

@@ -232,9 +232,14 @@ T lane_hairer(const LaneMol<T>& m, const T* x, const T* v, const T* a0, T rtol, 
   return std::min(std::min(T(100) * h0, h1), max_dt);
 }
 
+// One sim.  With t_cut > 0 (momentum-or-t_max rule) the run first stops at
+// t_cut as a run with t_max = t_cut would; a sim stopped there by the time
+// limit then continues to cfg->t_max from that state -- same x, v, E0, time
+// and counts, the FSAL force, and the step size the controller proposed after
+// the step that landed on t_cut (unclamped) -- as the engine's resume does.
 template <class T>
 void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, size_t sim,
-                  ceo_results* out) {
+                  ceo_results* out, double t_cut = 0.0) {
   const LaneCoef<T>& K = lane_coef<T>();
   const int n = m.n, n3 = 3 * m.n;
   const T rtol = static_cast<T>(cfg->rtol), atol = static_cast<T>(cfg->atol);
@@ -271,11 +276,13 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
     h = lane_hairer(m, x, v, a0, rtol, atol, max_dt);
   }
   const T inv6n = T(1) / static_cast<T>(6 * n);
+  bool cut_pending = t_cut > 0.0 && cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX;
   while (status == 255) {
+    const double t_lim = cut_pending ? t_cut : cfg->t_max;
     T cap = max_dt;
     bool by_time = false;
     if (cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX) {
-      const double rem = cfg->t_max - t;
+      const double rem = t_lim - t;
       if (rem <= cfg->max_dt) {
         cap = static_cast<T>(rem);
         by_time = true;
@@ -348,9 +355,13 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
     const T en2 = (ex_sum + ev_sum) * inv6n;
     // 0.9 err^(-1/5) with err = sqrt(en2): 0.9 * 2^(-0.1 log2 en2).
     const T grow = T(0.9) * std::exp2(T(-0.1) * std::log2(en2));
+    // The step that lands on the time limit (shortened to the remaining time, or
+    // ending there in FP64) is final; the momentum test is not applied to it.
+    const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= t_lim);
     if (en2 <= T(1)) {
       bool settled = true;
       if (cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX) {
+        settled = !final_step;
         for (int i = 0; i < n && settled; ++i) {
           const T dx = vn[3 * i] - v[3 * i], dy = vn[3 * i + 1] - v[3 * i + 1],
                   dz = vn[3 * i + 2] - v[3 * i + 2];
@@ -367,17 +378,20 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
       T f = (en2 == T(0)) ? T(10) : std::min(T(10), grow);
       if (rej_once) f = std::min(f, T(1));
-      h = std::min(hh * f, cap);
+      // After the step that lands on the time limit the proposal is kept
+      // unclamped (what a continuation from there starts with).
+      h = std::min(hh * f, final_step ? max_dt : cap);
       rej_once = false;
       nrej = 0;
       ++steps;
       trace.push_back(static_cast<double>(hh));
-      const bool final_step = by_time && hh >= cap;
-      t = final_step ? cfg->t_max : t + static_cast<double>(hh);
+      t = final_step ? t_lim : t + static_cast<double>(hh);
       if (cfg->stop_rule == CEO_STOP_PE_FRACTION) {
         if (static_cast<double>(lane_pe(m, x)) <= pe_stop) status = CEO_OK_PE;
       } else if (settled) {
         status = CEO_OK_MOMENTUM;
+      } else if (final_step && cut_pending) {
+        cut_pending = false;  // continue from the cut to cfg->t_max
       } else if (final_step) {
         status = CEO_OK_TMAX;
       }
@@ -424,12 +438,12 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
 
 template <class T>
 int lane_run(const ceo_molecule* cm, const ceo_config* cfg, const double* pos, size_t n_sims,
-             ceo_results* out, const double* charges) {
+             ceo_results* out, const double* charges, double t_cut = 0.0) {
   if (cm->n < 1 || cm->n > kMaxN) return -1;
   LaneMol<T> m = lane_mol<T>(cm, std::is_same<T, float>::value && charges == nullptr);
   for (size_t sim = 0; sim < n_sims; ++sim) {
     if (charges) set_lane_charges<T>(m, cm, charges + sim * cm->n);
-    lane_run_one<T>(m, cfg, pos + sim * cm->n * 3, sim, out);
+    lane_run_one<T>(m, cfg, pos + sim * cm->n * 3, sim, out, t_cut);
   }
   return 0;
 }
@@ -472,6 +486,14 @@ int ceo_lane_run_q_f32(const ceo_molecule* mol, const ceo_config* cfg, const dou
 int ceo_lane_run_q_f64(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
                        const double* charges, size_t n_sims, ceo_results* out) {
   return lane_run<double>(mol, cfg, pos, n_sims, out, charges);
+}
+
+int ceo_lane_run_cut(const ceo_molecule* mol, const ceo_config* cfg, const double* pos,
+                     const double* charges, size_t n_sims, double t_cut, int32_t precision,
+                     ceo_results* out) {
+  if (!(t_cut > 0.0) || cfg->stop_rule != CEO_STOP_MOMENTUM_OR_TMAX) return -1;
+  return precision == 1 ? lane_run<double>(mol, cfg, pos, n_sims, out, charges, t_cut)
+                        : lane_run<float>(mol, cfg, pos, n_sims, out, charges, t_cut);
 }
 
 void ceo_lane_coefficients(double* c, double* aa, double* b, double* e, double* ea) {

@@ -129,9 +129,9 @@ int cer_run(int n, const double* m, const double* q, double k, double rtol, doub
           const std::vector<Vec3> vold = s.velocities;
           const Real dt = integ->step(mol, force, s, cap);
           ++nsteps;
-          const bool final_step = by_time && dt >= cap;
+          const bool final_step = by_time && (dt >= cap || t + static_cast<double>(dt) >= t_max);
           t = final_step ? t_max : t + static_cast<double>(dt);
-          bool settled = true;
+          bool settled = !final_step;  // not tested on the step landing on t_max
           for (int i = 0; i < n && settled; ++i) {
             const Vec3 d = s.velocities[i] - vold[i];
             settled = dot(d, d) < tol2 * dot(s.velocities[i], s.velocities[i]);

@@ -327,9 +327,11 @@ int scalar_run(const ceo_molecule* cm, const ceo_config* cfg, const double* pos,
         }
         trace.push_back(dt);
         ++steps;
-        const bool final_step = cap_by_time && dt >= cap;
+        const bool final_step =
+            cap_by_time && (dt >= cap || t + static_cast<double>(dt) >= cfg->t_max);
         t = final_step ? cfg->t_max : t + static_cast<double>(dt);
-        if (momentum_settled(n, vold, v, tol2)) {
+        // The momentum test is not applied to the step that lands on t_max.
+        if (!final_step && momentum_settled(n, vold, v, tol2)) {
           status = CEO_OK_MOMENTUM;
           break;
         }

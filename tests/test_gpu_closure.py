@@ -60,6 +60,62 @@ def test_cut_then_resume_matches_uninterrupted(gpu, oracle):
     assert np.abs(res.steps.astype(int) - full.steps.astype(int)).max() <= 3
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_cut_resume_vs_oracle_cut(gpu, oracle, precision):
+    """A cut at t_cut followed by a resume to t_max against the oracle's statement
+    of the same continuation (ceo_lane_run_cut: the state at the cut, the FSAL
+    force, E0, time and counts carried over, and the step size proposed after the
+    step that landed on t_cut).  f64: the same step counts and <= 1e-12; f32:
+    <= 1e-5 and the same statuses.  A defect in what the resume restores moves
+    the f64 result far beyond 1e-12."""
+    ce = gpu
+    from oracle import ffi
+    p = ce.PRECISION_F64 if precision == "f64" else ce.PRECISION_F32
+    rtol, atol = (1e-8, 1e-12) if precision == "f64" else (1e-6, 1e-9)
+    eng = ce.BatchedRK45(ch4(ce), 1.0, 3000, rtol, atol)
+    eng.sample(77, 3000, 2.5, 1.0)
+    eng.enable_state()
+    cfg = ce.RunConfig(precision=p, **RULE)
+    cut = eng.run(replace(cfg, t_max=T_CUT))
+    res = eng.run(replace(cfg, resume=True))
+    ref = oracle.lane_run_cut(CH4_MASSES, CH4_CHARGES, eng.geometries(),
+                              ffi.make_config(rtol=rtol, atol=atol, **RULE), T_CUT,
+                              precision=precision)
+    assert (cut.status == ce.SIM_OK_TMAX).mean() > 0.99
+    np.testing.assert_array_equal(res.status, ref.status)
+    np.testing.assert_array_equal(res.t_final, ref.t_final)
+    err = per_atom_rel(res.momenta, ref.momenta)
+    if precision == "f64":
+        np.testing.assert_array_equal(res.steps, ref.steps)
+        np.testing.assert_array_equal(res.attempts, ref.attempts)
+        assert err.max() < 1e-12
+    else:
+        assert err.max() < 1e-5
+
+
+def test_clamped_step_at_the_cut_is_not_a_momentum_stop(gpu, oracle):
+    """CH4 sims whose last step before t_cut is clamped to a few 1e-5 a.u.: the
+    cut ends them by the time limit (not as settled), and continuing them gives
+    the uninterrupted result (these ended 0.03-0.12 off in the config norm while
+    the momentum test was applied to that shortened step)."""
+    ce = gpu
+    from test_oracle_cut import CLAMP_SEED, CLAMP_SIMS
+    geom = np.concatenate([oracle.device_sample(CLAMP_SEED, i, 1, 5, 2.5, 1.0)[0]
+                           for i in CLAMP_SIMS])
+    eng = ce.BatchedRK45(ch4(ce), 1.0, len(geom), 1e-6, 1e-9)
+    eng.set_geometries(geom)
+    eng.enable_state()
+    cfg = ce.RunConfig(**RULE)
+    full = eng.run(cfg)
+    cut = eng.run(replace(cfg, t_max=T_CUT))
+    assert np.all(cut.status == ce.SIM_OK_TMAX) and np.all(cut.t_final == T_CUT)
+    res = eng.run(replace(cfg, resume=True))
+    assert np.all(res.status == ce.SIM_OK_TMAX)
+    from conftest import config_rel
+    n = len(geom)
+    assert config_rel(res.momenta.reshape(n, 5, 3), full.momenta.reshape(n, 5, 3)).max() < 1e-6
+
+
 def test_resume_needs_state_of_same_precision(gpu):
     ce = gpu
     eng = engine(ce, 64, 1, state=False)
