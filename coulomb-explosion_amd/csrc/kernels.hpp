@@ -448,7 +448,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           sx[q] = x[q];
           sv[q] = v[q];
         }
-        p.state_h[gs] = static_cast<double>(h);
+        p.state_h[gs] = maxT(static_cast<double>(h), p.state_h[gs]);
         p.state_e0[gs] = e0d;
       }
       tot.att += p.resume ? att - p.attempts[gs] : att;
@@ -582,6 +582,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
             }
           }
         }
+        if (p.state_x != nullptr) p.state_h[sg.start + loc] = 0.0;  // (read above on a resume)
       }
     }
     if (__ballot(active) == 0) break;
@@ -600,6 +601,15 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
       }
       const T hh = minT(h, cap);
+      // With a saved state, a resume after the step that is shortened to land on
+      // the time limit starts from at least the size that step was shortened
+      // from (growing back from a step clamped to a sliver would take several
+      // steps, each a sliver-sized momentum change): the proposal of every
+      // shortened attempt goes to the sim's saved-step slot, zeroed when the sim
+      // starts and combined at the stop.  Memory, not a register kept through
+      // the step.
+      if (p.state_x != nullptr && by_time && h > cap)
+        p.state_h[sg.start + loc] = static_cast<double>(h);
       const T hh2 = hh * hh;
       T g[N];  // h^2 / m_i
 #pragma unroll
@@ -673,11 +683,6 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
       const T en2 = (exs + evs) * inv6n;
       // 0.9 err^(-1/5) with err = sqrt(en2): 0.9 * 2^(-0.1 log2 en2).
       const T grow = T(0.9) * exp2T(T(-0.1) * log2T(en2));
-      // The step that lands on t_max (shortened to the remaining time, or ending
-      // there in FP64) ends the sim by the time limit: the momentum test measures
-      // convergence only over steps the controller chose, so it is not applied
-      // to that one.
-      const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
       if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
@@ -691,7 +696,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
                                fmaT(vn[3 * i + 1], vn[3 * i + 1], vn[3 * i] * vn[3 * i]));
             return num < tol2 * den;
           };
-          settled = !final_step && atom_settled(0);
+          settled = atom_settled(0);
           if (__ballot(settled) != 0) {
 #pragma unroll
             for (int i = 1; i < N; ++i) settled = settled && atom_settled(i);
@@ -705,8 +710,12 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         }
         T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
-        // After the step that lands on t_max the proposal is kept unclamped: the
-        // sim is finished unless it is resumed from its saved state.
+        // The step that lands on t_max (shortened to the remaining time, or ending
+        // there in FP64) ends the sim by the time limit: the momentum test measures
+        // convergence only over steps the controller chose, so it does not count
+        // on that one.  After it the sim is finished unless it is resumed from its
+        // saved state; the step size kept for that is not clamped to the limit.
+        const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
         h = minT(hh * f, final_step ? max_dt : cap);
         rej_once = false;
         nrej = 0;
@@ -719,10 +728,10 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           if (static_cast<double>(lane_pe<T, N, CM>(x, pk)) <=
               p.pe_stop_fraction * static_cast<double>(e0))
             status = kSimOkPE;
-        } else if (settled) {
-          status = kSimOkMomentum;
         } else if (final_step) {
           status = kSimOkTmax;
+        } else if (settled) {
+          status = kSimOkMomentum;
         }
         if (status == kRunning && steps >= p.max_steps) status = kSimNotConverged;
       } else {
@@ -821,9 +830,12 @@ struct MixedConst {
   MolConst<T, N3> m3;
 };
 
+// f32: two waves per SIMD (the N = 8 segment's registers set the kernel's; its
+// one-lane pair-table form sits at the 256-register limit of two waves).
 template <class T, int N0, int N1, int N2, int N3, int CM>
-__global__ void __launch_bounds__(256) dp5_mixed(const RunArgs p,
-                                                 const MixedConst<T, N0, N1, N2, N3> mc) {
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, 256),
+                               amdgpu_waves_per_eu(std::is_same<T, float>::value ? 2 : 1)))
+dp5_mixed(const RunArgs p, const MixedConst<T, N0, N1, N2, N3> mc) {
   Totals tot;
   if (p.n_segments > 0) run_any<T, N0, CM>(p, p.seg[0], mc.m0, p.ctr + kCtrQueue + 0, tot);
   if (p.n_segments > 1) run_any<T, N1, CM>(p, p.seg[1], mc.m1, p.ctr + kCtrQueue + 1, tot);

@@ -320,7 +320,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
           sv[q] = v[q];
         }
         if (lead) {
-          p.state_h[gs] = static_cast<double>(h);
+          p.state_h[gs] = maxT(static_cast<double>(h), p.state_h[gs]);
           p.state_e0[gs] = e0d;
         }
       }
@@ -452,6 +452,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
             h = pair_hairer<NA, CM>(x, v, a0, im, rtol, atol, max_dt, pk);
           }
         }
+        if (p.state_x != nullptr && lead) p.state_h[gs] = 0.0;  // (read above on a resume)
       }
     }
     if (__ballot(active) == 0) break;
@@ -470,6 +471,10 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         }
       }
       const T hh = minT(h, cap);
+      // the proposal of a shortened attempt to the saved-step slot (as in the
+      // one-lane form; the lead lane keeps the sim's per-sim values)
+      if (p.state_x != nullptr && lead && by_time && h > cap)
+        p.state_h[sg.start + loc] = static_cast<double>(h);
       const T hh2 = hh * hh;
       T g[NA];
 #pragma unroll
@@ -522,9 +527,6 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
       }
       const T en2 = (pair_sum(exs) + pair_sum(evs)) * inv6n;
       const T grow = T(0.9) * exp2T(T(-0.1) * log2T(en2));
-      // The step that lands on t_max ends the sim by the time limit; the momentum
-      // test is not applied to it (its change covers a shortened interval).
-      const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
       if (en2 <= T(1)) {
         bool settled = true;
         if (p.stop_rule == kStopMomentumOrTmax) {
@@ -537,7 +539,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
             return num < tol2 * den;
           };
           // own atom 0 first, the rest only where some pair of the wave passes it
-          int ok = (!final_step && atom_settled(0)) ? 1 : 0;
+          int ok = atom_settled(0) ? 1 : 0;
           ok = ok & xchg(ok);
           if (__ballot(ok != 0) != 0) {
 #pragma unroll
@@ -554,6 +556,9 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         }
         T f = (en2 == T(0)) ? T(10) : minT(T(10), grow);
         if (rej_once) f = minT(f, T(1));
+        // The step that lands on t_max ends the sim by the time limit; the momentum
+        // test does not count on it (its change covers a shortened interval).
+        const bool final_step = by_time && (hh >= cap || t + static_cast<double>(hh) >= p.t_max);
         h = minT(hh * f, final_step ? max_dt : cap);
         rej_once = false;
         nrej = 0;
@@ -566,10 +571,10 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
           if (static_cast<double>(pair_pe<NA, CM>(x, pk)) <=
               p.pe_stop_fraction * static_cast<double>(e0))
             status = kSimOkPE;
-        } else if (settled) {
-          status = kSimOkMomentum;
         } else if (final_step) {
           status = kSimOkTmax;
+        } else if (settled) {
+          status = kSimOkMomentum;
         }
         if (status == kRunning && steps >= p.max_steps) status = kSimNotConverged;
       } else {

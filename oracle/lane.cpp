@@ -277,6 +277,11 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
   }
   const T inv6n = T(1) / static_cast<T>(6 * n);
   bool cut_pending = t_cut > 0.0 && cfg->stop_rule == CEO_STOP_MOMENTUM_OR_TMAX;
+  // A continuation from a stop starts from at least the proposal of the run's
+  // last attempt that was shortened to land on the time limit (the engine's
+  // saved step size): growing back from a step clamped to a sliver would take
+  // several steps, each a sliver-sized momentum change.
+  T h_short = T(0);
   while (status == 255) {
     const double t_lim = cut_pending ? t_cut : cfg->t_max;
     T cap = max_dt;
@@ -289,6 +294,7 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
     }
     const T hh = std::min(h, cap);
+    if (by_time && h > cap) h_short = h;  // proposal of the last shortened attempt
     const T hh2 = hh * hh;
     T g[kMaxN], gv[kMaxN];  // h^2 / m_i and h / m_i
     for (int i = 0; i < n; ++i) {
@@ -378,8 +384,8 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
       }
       T f = (en2 == T(0)) ? T(10) : std::min(T(10), grow);
       if (rej_once) f = std::min(f, T(1));
-      // After the step that lands on the time limit the proposal is kept
-      // unclamped (what a continuation from there starts with).
+      // After the step that lands on the time limit the step size a continuation
+      // starts with is not clamped to the limit.
       h = std::min(hh * f, final_step ? max_dt : cap);
       rej_once = false;
       nrej = 0;
@@ -392,6 +398,8 @@ void lane_run_one(const LaneMol<T>& m, const ceo_config* cfg, const double* p0, 
         status = CEO_OK_MOMENTUM;
       } else if (final_step && cut_pending) {
         cut_pending = false;  // continue from the cut to cfg->t_max
+        h = std::max(h, h_short);
+        h_short = T(0);
       } else if (final_step) {
         status = CEO_OK_TMAX;
       }

@@ -50,3 +50,43 @@ def config_rel(p, ref):
     p = np.asarray(p).reshape(ref.shape)
     d = (p - ref).reshape(ref.shape[0], -1)
     return np.linalg.norm(d, axis=1) / np.linalg.norm(ref.reshape(ref.shape[0], -1), axis=1)
+
+
+def f32_sensitivity(oracle, masses, charges, geom, cfg64, n_perturb=6, seed=0):
+    """How far rounding-sized changes of the start move one sim's answer: the
+    largest per-atom relative change of its f64 final momenta (oracle per-lane
+    form, config cfg64) when every start coordinate is scaled by 1 +/- 6e-8
+    (random signs; one f32 rounding), over n_perturb draws."""
+    q = np.asarray(charges, dtype=np.float64)[None]
+    g = np.asarray(geom, dtype=np.float64)
+    base = oracle.lane_run_q(masses, q, g[None], cfg64, precision="f64").momenta
+    rng = np.random.default_rng(seed)
+    out = 0.0
+    for _ in range(n_perturb):
+        gp = g * (1 + 6e-8 * rng.choice([-1.0, 1.0], size=g.shape))
+        p = oracle.lane_run_q(masses, q, gp[None], cfg64, precision="f64").momenta
+        out = max(out, float(per_atom_rel(p, base)[0]))
+    return out
+
+
+def assert_f32_bar(oracle, masses, charges, geoms, got, ref, cfg64):
+    """The f32 parity bar.  Every sim within 1e-5 of `ref` in the config norm
+    |dP|/|P| (the reference's metric, examples/verify_subset.py:77-79), and per
+    atom within 1e-5 -- except sims whose own conditioning is the limit: there
+    (at most 0.5% of the sims) the per-atom error is at most 8x f32_sensitivity,
+    the change that one f32 rounding of the start causes in the f64 trajectory.
+    (Measured on 80k N = 8 / N = 10 sims: config norm <= 8.1e-6 everywhere;
+    per-atom outliers 0.005-0.035% of sims, each within 5.6x its sensitivity;
+    the reference's own f32 engine misses by as much on them.)  `charges` is
+    [sims][atoms]."""
+    n = len(geoms)
+    got = np.asarray(got).reshape(np.shape(ref))
+    err = per_atom_rel(got, ref)
+    cerr = config_rel(got, ref)
+    assert cerr.max() <= 1e-5, ("config norm", float(cerr.max()))
+    out = np.nonzero(err > 1e-5)[0]
+    assert len(out) <= max(2, int(0.005 * n)), ("per-atom outliers", len(out), float(err.max()))
+    for s in out:
+        sens = f32_sensitivity(oracle, masses, charges[s], geoms[s], cfg64)
+        assert err[s] <= 8 * sens, ("per atom beyond the sim's conditioning", int(s),
+                                    float(err[s]), sens)

@@ -100,8 +100,10 @@ def test_clamped_step_at_the_cut_is_not_a_momentum_stop(gpu, oracle):
     the momentum test was applied to that shortened step)."""
     ce = gpu
     from test_oracle_cut import CLAMP_SEED, CLAMP_SIMS
+    # + 6147900: its GPU cut ends with a sliver, and the old continuation (10x the
+    # sliver) stopped it "settled" at t = 200.00005
     geom = np.concatenate([oracle.device_sample(CLAMP_SEED, i, 1, 5, 2.5, 1.0)[0]
-                           for i in CLAMP_SIMS])
+                           for i in CLAMP_SIMS + [6147900]])
     eng = ce.BatchedRK45(ch4(ce), 1.0, len(geom), 1e-6, 1e-9)
     eng.set_geometries(geom)
     eng.enable_state()

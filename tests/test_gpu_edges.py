@@ -206,3 +206,44 @@ def test_forced_failures_end_sims_with_their_status(gpu, oracle):
         assert np.all(r.converged == 0), name
         np.testing.assert_array_equal(r.status, o.status)
         np.testing.assert_array_equal(r.attempts, o.attempts)
+
+
+@pytest.mark.parametrize("per_sim_charges", [False, True])
+@pytest.mark.parametrize("h0", [20.0, 100.0])
+def test_first_force_of_a_fresh_sim_under_first_step_rejections(gpu, oracle, h0,
+                                                                 per_sim_charges):
+    """A fresh CH4 sim takes its first force from the second stage of its first
+    attempt (it starts at rest, so that stage sits at x0).  With a fixed h0 large
+    enough to be rejected once or more, that attempt is a rejected one, with the
+    molecule's charges and with per-sim charges: the f64 engine still takes the
+    oracle's steps and attempts and agrees to <= 1e-12, and f32 to <= 1e-5."""
+    ce = gpu
+    from oracle import ffi
+    n = 1500
+    rule = dict(RULE, h0=h0)
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in CH4_MASSES])
+    for precision, rtol, atol in ((1, 1e-10, 1e-14), (0, 1e-6, 1e-9)):
+        if per_sim_charges:
+            eng = ce.MixedBatch([ce.Species(mol, n, 2.5, 1.0, charge_levels=2,
+                                            max_total_charge=10)], 1.0, rtol, atol)
+            eng.sample(seed=61)
+            g, q = eng.geometries()[0], eng.charges()[0]
+            r = eng.run(ce.RunConfig(precision=precision, **rule))[0]
+            mom, steps, att = r["momenta"], r["steps"], r["attempts"]
+        else:
+            eng = ce.BatchedRK45(mol, 1.0, n, rtol, atol)
+            eng.sample(61, n, 2.5, 1.0)
+            g, q = eng.geometries(), np.ones((n, 5))
+            r = eng.run(ce.RunConfig(precision=precision, **rule))
+            mom, steps, att = r.momenta.reshape(n, 5, 3), r.steps, r.attempts
+        ref = oracle.lane_run_q(CH4_MASSES, q, g, ffi.make_config(rtol=rtol, atol=atol, **rule),
+                                precision="f64" if precision == 1 else "f32")
+        assert np.mean(att > steps) > 0.9  # (nearly) every sim rejects its first attempt
+        err = (np.linalg.norm(mom - ref.momenta, axis=-1) /
+               np.linalg.norm(ref.momenta, axis=-1)).max(axis=-1)
+        if precision == 1:
+            np.testing.assert_array_equal(steps, ref.steps)
+            np.testing.assert_array_equal(att, ref.attempts)
+            assert err.max() < 1e-12
+        else:
+            assert err.max() < 1e-5

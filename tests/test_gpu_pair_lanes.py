@@ -2,14 +2,15 @@
 two lanes) on the paths the headline rule does not take: the reference's PE rule
 with the Hairer start, a stop at max_steps resumed from the saved state, the
 step-size trace, per-sim charges with the energy diagnostic, and sampler
-failures.  References: the oracle's per-lane form (f32, <= 1e-5 per atom for
->= 99% of sims; an 8-atom f32 trajectory is only good to ~1e-4 on an odd sim)."""
+failures.  References: the oracle's per-lane form, to the f32 bar of
+conftest.assert_f32_bar (config norm <= 1e-5 for every sim; per atom <= 1e-5
+except where the sim's own conditioning allows more)."""
 from dataclasses import replace
 
 import numpy as np
 import pytest
 
-from conftest import CH4_MASSES, per_atom_rel
+from conftest import CH4_MASSES, assert_f32_bar, per_atom_rel
 
 pytestmark = pytest.mark.gpu
 
@@ -22,27 +23,42 @@ def c2h6(ce, q=1.0):
     return ce.Molecule([ce.Atom("X", m, q) for m in C2H6_MASSES])
 
 
-def f32_close(res_momenta, ref_momenta):
-    err = per_atom_rel(res_momenta, ref_momenta)
-    return np.mean(err < 1e-5) >= 0.99 and err.max() < 1e-4, err.max()
-
-
-def test_pe_rule_and_hairer_start_vs_oracle(gpu, oracle):
+@pytest.mark.parametrize("pe_stop", [1e-4, 1e-5, 1e-9])
+def test_pe_rule_and_hairer_start_vs_oracle(gpu, oracle, pe_stop):
+    """The reference's PE rule with the Hairer start at its production thresholds
+    (1e-4, 1e-5: docs/decisions/0004-precision-policy.md:42-46) and its default
+    (1e-9).  In f32 the early error estimates are rounding noise, so two
+    arithmetic arrangements take different step sequences from the second step on
+    (the oracle's per-lane and reference forms do too) and stop a step apart on a
+    tenth of the sims; at 1e-4 / 1e-5 a stop is only as close to the asymptote as
+    the threshold allows (config norm ~1.5e-4 / ~1.6e-5 for every form, f64
+    included).  So there: every sim stops by the rule, and each GPU sim is as
+    close to the f64 asymptote as the worst sim of the oracle's f32 per-lane form
+    (x1.5), with the medians within 25%.  At 1e-9 the momenta are compared sim by
+    sim with the f32 bar."""
     from oracle import ffi
+    from conftest import config_rel
     ce = gpu
-    _, geom = oracle.sample(81, 8, 300, R8, SEP8)
-    eng = ce.BatchedRK45(c2h6(ce), 1.0, 300, 1e-6, 1e-9)
+    n = 300
+    _, geom = oracle.sample(81, 8, n, R8, SEP8)
+    eng = ce.BatchedRK45(c2h6(ce), 1.0, n, 1e-6, 1e-9)
     eng.set_geometries(geom)
-    # the reference's default threshold: at PE = 1e-9 PE0 a step more or less
-    # changes the momenta by far less than the f32 bar (at 1e-4 it would not)
-    cfg = dict(stop_rule=0, pe_stop_fraction=1e-9)
-    res = eng.run(ce.RunConfig(**cfg))
-    ref = oracle.lane_run(C2H6_MASSES, [1.0] * 8, geom,
-                          ffi.make_config(rtol=1e-6, atol=1e-9, **cfg), precision="f32")
-    ok, worst = f32_close(res.momenta, ref.momenta)
-    assert ok, worst
-    assert np.mean(res.status == ref.status) > 0.99
+    rule = dict(stop_rule=0, pe_stop_fraction=pe_stop)
+    res = eng.run(ce.RunConfig(**rule))
     assert np.all(res.status == ce.SIM_OK_PE)
+    ref = oracle.lane_run(C2H6_MASSES, [1.0] * 8, geom,
+                          ffi.make_config(rtol=1e-6, atol=1e-9, **rule), precision="f32")
+    got = res.momenta.reshape(n, 8, 3)
+    if pe_stop < 1e-6:
+        assert_f32_bar(oracle, C2H6_MASSES, np.ones((n, 8)), geom, got, ref.momenta,
+                       ffi.make_config(rtol=1e-10, atol=1e-14, **rule))
+        return
+    asym = oracle.lane_run(C2H6_MASSES, [1.0] * 8, geom,
+                           ffi.make_config(rtol=1e-11, atol=1e-15, stop_rule=0,
+                                           pe_stop_fraction=1e-12), precision="f64").momenta
+    e_gpu, e_ref = config_rel(got, asym), config_rel(ref.momenta, asym)
+    assert e_gpu.max() <= 1.5 * e_ref.max(), (e_gpu.max(), e_ref.max())
+    assert 0.8 < np.median(e_gpu) / np.median(e_ref) < 1.25
 
 
 def test_max_steps_stop_then_resume_gives_the_uninterrupted_bits(gpu):
@@ -89,8 +105,8 @@ def test_per_sim_charges_and_energy_diagnostic_vs_oracle(gpu, oracle):
     g, q = mb.geometries()[0], mb.charges()[0]
     ref = oracle.lane_run_q(np.array(C2H6_MASSES), q, g,
                             ffi.make_config(rtol=1e-6, atol=1e-9, **RULE), precision="f32")
-    ok, worst = f32_close(r["momenta"], ref.momenta)
-    assert ok, worst
+    assert_f32_bar(oracle, C2H6_MASSES, q, g, r["momenta"], ref.momenta,
+                   ffi.make_config(rtol=1e-10, atol=1e-14, **RULE))
     assert np.mean(r["status"] == ref.status) > 0.99
     de = np.abs(r["energy_err"] - ref.energy_err)
     assert np.median(de) < 1e-6 and de.max() < 2e-5, (np.median(de), de.max())

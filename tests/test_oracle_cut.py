@@ -8,12 +8,17 @@ statement (ceo_lane_run_cut), on the CPU.
   (8 of 8M CH4 sims of the closure workload, config-norm error up to 0.12); they
   are the regression cases here.
 * The continuation is pinned against the uninterrupted run: with t_cut placed
-  exactly where the uninterrupted run's k-th step ends, cutting there and
-  continuing with the step size the controller proposed after that step
-  reproduces the uninterrupted run (same step count and time, <= 1e-12 in f64),
-  which is itself pinned to the reference engine.  Continuing with the clamped
-  step size instead (what the reference integrator would keep after a capped
-  step) changes the step sequence and fails this.
+  exactly where the uninterrupted run's k-th step ends (a step after which the
+  controller grew the step), cutting there and continuing with the step size
+  the controller proposed after that step reproduces the uninterrupted run (same
+  step count and time, <= 1e-12 in f64), which is itself pinned to the
+  reference engine.  Continuing with the clamped step size instead (what the
+  reference integrator would keep after a capped step) changes the step
+  sequence and fails this.
+* A cut a sliver after a step boundary clamps the last step to a sliver; the
+  continuation starts from at least the size that step was shortened from, so
+  it does not crawl back from the sliver with sliver-sized momentum changes
+  (which pass the momentum test: sims stopped "settled" just after the cut).
 """
 import numpy as np
 import pytest
@@ -67,6 +72,8 @@ def test_cut_at_a_step_boundary_reproduces_uninterrupted_run(oracle):
         if full.attempts[s] != full.steps[s]:
             continue  # a rejected proposal near the cut would be clamped differently
         k = int(full.steps[s]) // 2
+        while k < full.steps[s] - 2 and not full.trace_h[s, k] > full.trace_h[s, k - 1]:
+            k += 1  # a step after which the controller grew the step (f > 1)
         t_cut = 0.0
         for h in full.trace_h[s, :k]:  # the engine's FP64 time, summed step by step
             t_cut += h
@@ -90,3 +97,32 @@ def test_cut_rejects_other_rules(oracle):
         oracle.lane_run_cut(CH4_MASSES, CH4_CHARGES, g, ffi.make_config(), 200.0)
     with pytest.raises(RuntimeError):
         oracle.lane_run_cut(CH4_MASSES, CH4_CHARGES, g, ffi.make_config(**RULE), 0.0)
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_cut_just_after_a_step_boundary_continues_normally(oracle, precision):
+    """t_cut a sliver after the end of step k (1e-6 a.u.; 3e-5 in f32, whose
+    step cap rounds to ~2e-6 at these step sizes): the cut's last step is clamped
+    to the sliver; the continuation starts from the size that step was shortened
+    from and ends by the time limit, agreeing with the uninterrupted run to the
+    tolerance."""
+    g, _, _ = oracle.device_sample(5151, 0, 16, 5, 2.5, 1.0)
+    rtol, atol = (1e-6, 1e-9) if precision == "f32" else (1e-8, 1e-12)
+    cfg = ffi.make_config(rtol=rtol, atol=atol, **RULE)
+    full = oracle.lane_run(CH4_MASSES, CH4_CHARGES, g, cfg, precision=precision, trace_cap=256)
+    for s in range(len(g)):
+        k = int(full.steps[s]) // 2
+        t_cut = 0.0
+        for h in full.trace_h[s, :k]:
+            t_cut += h
+        t_cut += 1e-6 if precision == "f64" else 3e-5
+        cut = oracle.lane_run(CH4_MASSES, CH4_CHARGES, g[s:s + 1],
+                              ffi.make_config(rtol=rtol, atol=atol, **dict(RULE, t_max=t_cut)),
+                              precision=precision, trace_cap=256)
+        assert cut.status[0] == ffi.OK_TMAX
+        assert cut.trace_h[0, cut.steps[0] - 1] < 1e-4  # the sliver
+        one = oracle.lane_run_cut(CH4_MASSES, CH4_CHARGES, g[s:s + 1], cfg, t_cut,
+                                  precision=precision, trace_cap=256)
+        assert one.status[0] == ffi.OK_TMAX and one.t_final[0] == 1e4
+        assert one.trace_h[0, cut.steps[0]] > 1.0  # no crawl back from the sliver
+        assert config_rel(one.momenta, full.momenta[s:s + 1]).max() < 1e-6

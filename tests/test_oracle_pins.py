@@ -249,3 +249,24 @@ def test_charges_split_invariant_and_positions_independent(oracle):
     np.testing.assert_array_equal(plain, a_g)
     c_g, c_q, _, _ = oracle.device_sample_q(18, 0, 100, 8, 3.5, 0.9, 3, 20)
     assert not np.array_equal(a_q, c_q)
+
+
+def test_f32_bar_admits_conditioning_and_rejects_planted_error(oracle):
+    """The GPU tests' f32 bar (conftest.assert_f32_bar) on the oracle's own f32
+    per-lane form against f64, N = 10 with charges {1,2,3}: the slice holds a
+    sim (1071) beyond 1e-5 per atom whose start is that ill-conditioned, and it
+    passes; a 2e-5 error planted on a well-conditioned sim fails."""
+    from conftest import assert_f32_bar
+    masses = [21874.66] * 2 + [29156.95] * 2 + [1837.15] * 6
+    rule = dict(stop_rule=1, t_max=1e4, momentum_tol=1e-7, h0_mode=1, h0=1e-3)
+    g, q, st, fails = oracle.device_sample_q(49188291, 1000, 300, 10, 3.5, 0.9, 3, 0)
+    cfg64 = ffi.make_config(rtol=1e-10, atol=1e-14, **rule)
+    p32 = oracle.lane_run_q(masses, q, g, ffi.make_config(rtol=1e-6, atol=1e-9, **rule),
+                            precision="f32").momenta
+    p64 = oracle.lane_run_q(masses, q, g, cfg64, precision="f64").momenta
+    assert per_atom_rel(p32, p64)[71] > 1e-5
+    assert_f32_bar(oracle, masses, q, g, p32, p64, cfg64)
+    bad = p32.copy()
+    bad[5, 2] *= 1 + 2e-5
+    with pytest.raises(AssertionError):
+        assert_f32_bar(oracle, masses, q, g, bad, p64, cfg64)
