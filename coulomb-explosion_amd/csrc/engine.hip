@@ -234,6 +234,90 @@ int copy_results(ce_engine* e, ce_results* out) {
   return CE_OK;
 }
 
+// The reference-layout force and potential-energy kernels over host buffers
+// (one allocation per call; not on the engine's path).
+template <class T>
+int soa_accelerations(size_t n, size_t k, const T* px, const T* py, const T* pz,
+                      const T* pair_const, const T* inv_mass, T* ax, T* ay, T* az) {
+  if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n out of range");
+  if (!px || !py || !pz || !pair_const || !inv_mass || !ax || !ay || !az)
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (k == 0) return CE_OK;
+  if (k > SIZE_MAX / (8 * sizeof(T) * CE_MAX_ATOMS))
+    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
+  const size_t nk = n * k;
+  T* d = nullptr;
+  const size_t total = 6 * nk + n * n + n;
+  CE_HIP(hipMalloc(&d, total * sizeof(T)));
+  T *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dax = d + 3 * nk, *day = d + 4 * nk,
+    *daz = d + 5 * nk, *dpc = d + 6 * nk, *dim = dpc + n * n;
+  int rc = CE_OK;
+  auto chk = [&](hipError_t h) {
+    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+  };
+  chk(hipMemcpy(dx, px, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dy, py, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dz, pz, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dpc, pair_const, n * n * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dim, inv_mass, n * sizeof(T), hipMemcpyHostToDevice));
+  if (rc == CE_OK) {
+    const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
+    switch (n) {
+#define CE_ACC(NN)                                                                           \
+  case NN:                                                                                   \
+    hipLaunchKernelGGL((ce::soa_accel<T, NN>), grid, block, 0, nullptr, k, dx, dy, dz, dpc,  \
+                       dim, dax, day, daz);                                                  \
+    break;
+      CE_ACC(1) CE_ACC(2) CE_ACC(3) CE_ACC(4) CE_ACC(5) CE_ACC(6) CE_ACC(7) CE_ACC(8) CE_ACC(9)
+      CE_ACC(10)
+#undef CE_ACC
+    }
+    chk(hipGetLastError());
+    chk(hipMemcpy(ax, dax, nk * sizeof(T), hipMemcpyDeviceToHost));
+    chk(hipMemcpy(ay, day, nk * sizeof(T), hipMemcpyDeviceToHost));
+    chk(hipMemcpy(az, daz, nk * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+template <class T>
+int soa_potential_energy(size_t n, size_t k, const T* px, const T* py, const T* pz,
+                         const T* pair_const, T* pe) {
+  if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n out of range");
+  if (!px || !py || !pz || !pair_const || !pe) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (k == 0) return CE_OK;
+  if (k > SIZE_MAX / (8 * sizeof(T) * CE_MAX_ATOMS))
+    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
+  const size_t nk = n * k;
+  T* d = nullptr;
+  CE_HIP(hipMalloc(&d, (3 * nk + n * n + k) * sizeof(T)));
+  T *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dpc = d + 3 * nk, *dpe = dpc + n * n;
+  int rc = CE_OK;
+  auto chk = [&](hipError_t h) {
+    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+  };
+  chk(hipMemcpy(dx, px, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dy, py, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dz, pz, nk * sizeof(T), hipMemcpyHostToDevice));
+  chk(hipMemcpy(dpc, pair_const, n * n * sizeof(T), hipMemcpyHostToDevice));
+  if (rc == CE_OK) {
+    const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
+    switch (n) {
+#define CE_PE(NN)                                                                               \
+  case NN:                                                                                      \
+    hipLaunchKernelGGL((ce::soa_pe<T, NN>), grid, block, 0, nullptr, k, dx, dy, dz, dpc, dpe); \
+    break;
+      CE_PE(1) CE_PE(2) CE_PE(3) CE_PE(4) CE_PE(5) CE_PE(6) CE_PE(7) CE_PE(8) CE_PE(9) CE_PE(10)
+#undef CE_PE
+    }
+    chk(hipGetLastError());
+    chk(hipMemcpy(pe, dpe, k * sizeof(T), hipMemcpyDeviceToHost));
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
 template <int N>
 void launch_sample(const ce::SampleArgs& a, hipStream_t s) {
   const int block = 256;
@@ -640,6 +724,59 @@ int ce_engine_enable_state(ce_engine* e, int32_t enable) {
   return CE_OK;
 }
 
+int ce_engine_set_state(ce_engine* e, const double* pos, const double* vel, const double* h,
+                        const double* e0, size_t n_sims, int32_t precision) {
+  if (e == nullptr || ((pos == nullptr || vel == nullptr) && n_sims > 0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (precision != CE_PRECISION_F32 && precision != CE_PRECISION_F64)
+    return fail(CE_ERR_INVALID_ARGUMENT, "unknown precision");
+  if (e->mixed) {
+    if (n_sims != e->capacity)
+      return fail(CE_ERR_INVALID_ARGUMENT, "a mixed engine takes all its sims at once");
+  } else if (n_sims > e->capacity) {
+    return fail(CE_ERR_INVALID_ARGUMENT, "n_sims exceeds capacity");
+  }
+  int rc = ce_engine_enable_state(e, 1);
+  if (rc != CE_OK) return rc;
+  CE_HIP(hipSetDevice(e->device));
+  if (!e->mixed) e->sp[0].count = n_sims;
+  size_t n_pos = 0;
+  for (int s = 0; s < e->n_species; ++s) n_pos += e->sp[s].count * 3 * e->sp[s].n;
+  const bool f32 = precision == CE_PRECISION_F32;
+  const size_t el = f32 ? sizeof(float) : sizeof(double);
+  const double* src[2] = {pos, vel};
+  void* dst[2] = {e->state_x, e->state_v};
+  for (int b = 0; b < 2; ++b) {
+    std::vector<char> raw(n_pos * el);
+    for (size_t i = 0; i < n_pos; ++i) {
+      if (f32)
+        reinterpret_cast<float*>(raw.data())[i] = static_cast<float>(src[b][i]);
+      else
+        reinterpret_cast<double*>(raw.data())[i] = src[b][i];
+    }
+    rc = copy_per_atom(e, raw.data(), dst[b], el, 3, hipMemcpyHostToDevice);
+    if (rc != CE_OK) return rc;
+    CE_HIP(hipStreamSynchronize(e->stream));
+  }
+  std::vector<double> zero(e->loaded(), 0.0);
+  rc = copy_per_sim(e, const_cast<double*>(h ? h : zero.data()), e->state_h, sizeof(double),
+                    hipMemcpyHostToDevice);
+  if (rc == CE_OK)
+    rc = copy_per_sim(e, const_cast<double*>(e0 ? e0 : zero.data()), e->state_e0, sizeof(double),
+                      hipMemcpyHostToDevice);
+  if (rc == CE_OK) rc = copy_per_sim(e, zero.data(), e->tfin, sizeof(double), hipMemcpyHostToDevice);
+  std::vector<uint32_t> zero32(e->loaded(), 0u);
+  if (rc == CE_OK) rc = copy_per_sim(e, zero32.data(), e->steps, sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (rc == CE_OK) rc = copy_per_sim(e, zero32.data(), e->attempts, sizeof(uint32_t), hipMemcpyHostToDevice);
+  const hipError_t se = hipStreamSynchronize(e->stream);
+  if (rc != CE_OK) return rc;
+  if (se != hipSuccess) return fail(CE_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(se));
+  e->state_prec = precision;
+  e->sampled = false;
+  e->have_results = false;
+  return CE_OK;
+}
+
 int ce_engine_get_state(ce_engine* e, double* pos, double* vel, double* h, double* e0,
                         size_t n_sims) {
   if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
@@ -935,82 +1072,23 @@ int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t*
 int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const float* py,
                                  const float* pz, const float* pair_const, const float* inv_mass,
                                  float* ax, float* ay, float* az) {
-  if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n out of range");
-  if (!px || !py || !pz || !pair_const || !inv_mass || !ax || !ay || !az)
-    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (k == 0) return CE_OK;
-  if (k > SIZE_MAX / (8 * sizeof(float) * CE_MAX_ATOMS))
-    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
-  const size_t nk = n * k;
-  float* d = nullptr;
-  const size_t total = 6 * nk + n * n + n;
-  CE_HIP(hipMalloc(&d, total * sizeof(float)));
-  float *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dax = d + 3 * nk, *day = d + 4 * nk,
-        *daz = d + 5 * nk, *dpc = d + 6 * nk, *dim = dpc + n * n;
-  int rc = CE_OK;
-  auto chk = [&](hipError_t h) {
-    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
-  };
-  chk(hipMemcpy(dx, px, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dy, py, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dz, pz, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dpc, pair_const, n * n * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dim, inv_mass, n * sizeof(float), hipMemcpyHostToDevice));
-  if (rc == CE_OK) {
-    const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
-    switch (n) {
-#define CE_ACC(NN)                                                                        \
-  case NN:                                                                                \
-    hipLaunchKernelGGL(ce::soa_accel<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dim, \
-                       dax, day, daz);                                                    \
-    break;
-      CE_ACC(1) CE_ACC(2) CE_ACC(3) CE_ACC(4) CE_ACC(5) CE_ACC(6) CE_ACC(7) CE_ACC(8) CE_ACC(9)
-      CE_ACC(10)
-#undef CE_ACC
-    }
-    chk(hipGetLastError());
-    chk(hipMemcpy(ax, dax, nk * sizeof(float), hipMemcpyDeviceToHost));
-    chk(hipMemcpy(ay, day, nk * sizeof(float), hipMemcpyDeviceToHost));
-    chk(hipMemcpy(az, daz, nk * sizeof(float), hipMemcpyDeviceToHost));
-  }
-  (void)hipFree(d);
-  return rc;
+  return soa_accelerations<float>(n, k, px, py, pz, pair_const, inv_mass, ax, ay, az);
+}
+
+int ce_batched_accelerations_f64(size_t n, size_t k, const double* px, const double* py,
+                                 const double* pz, const double* pair_const,
+                                 const double* inv_mass, double* ax, double* ay, double* az) {
+  return soa_accelerations<double>(n, k, px, py, pz, pair_const, inv_mass, ax, ay, az);
 }
 
 int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const float* py,
                                     const float* pz, const float* pair_const, float* pe) {
-  if (n < 1 || n > CE_MAX_ATOMS) return fail(CE_ERR_INVALID_ARGUMENT, "n out of range");
-  if (!px || !py || !pz || !pair_const || !pe) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
-  if (k == 0) return CE_OK;
-  if (k > SIZE_MAX / (8 * sizeof(float) * CE_MAX_ATOMS))
-    return fail(CE_ERR_INVALID_ARGUMENT, "k too large");
-  const size_t nk = n * k;
-  float* d = nullptr;
-  CE_HIP(hipMalloc(&d, (3 * nk + n * n + k) * sizeof(float)));
-  float *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dpc = d + 3 * nk, *dpe = dpc + n * n;
-  int rc = CE_OK;
-  auto chk = [&](hipError_t h) {
-    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
-  };
-  chk(hipMemcpy(dx, px, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dy, py, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dz, pz, nk * sizeof(float), hipMemcpyHostToDevice));
-  chk(hipMemcpy(dpc, pair_const, n * n * sizeof(float), hipMemcpyHostToDevice));
-  if (rc == CE_OK) {
-    const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
-    switch (n) {
-#define CE_PE(NN)                                                                              \
-  case NN:                                                                                     \
-    hipLaunchKernelGGL(ce::soa_pe<NN>, grid, block, 0, nullptr, k, dx, dy, dz, dpc, dpe); \
-    break;
-      CE_PE(1) CE_PE(2) CE_PE(3) CE_PE(4) CE_PE(5) CE_PE(6) CE_PE(7) CE_PE(8) CE_PE(9) CE_PE(10)
-#undef CE_PE
-    }
-    chk(hipGetLastError());
-    chk(hipMemcpy(pe, dpe, k * sizeof(float), hipMemcpyDeviceToHost));
-  }
-  (void)hipFree(d);
-  return rc;
+  return soa_potential_energy<float>(n, k, px, py, pz, pair_const, pe);
+}
+
+int ce_batched_potential_energy_f64(size_t n, size_t k, const double* px, const double* py,
+                                    const double* pz, const double* pair_const, double* pe) {
+  return soa_potential_energy<double>(n, k, px, py, pz, pair_const, pe);
 }
 
 }  // extern "C"

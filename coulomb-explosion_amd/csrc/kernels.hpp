@@ -550,6 +550,18 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
           T unused;
           lane_force<T, N, CM, false, false>(x, k[0], unused, pk);
           if (steps >= p.max_steps) status = kSimNotConverged;
+          if (!(h > T(0))) {
+            // a given state without a step size (ce_engine_set_state) starts its
+            // step size like a fresh sim
+            if (p.h0_mode == kH0Fixed) {
+              h = static_cast<T>(p.h0);
+            } else {
+              T a0[N3];
+#pragma unroll
+              for (int q = 0; q < N3; ++q) a0[q] = k[0][q] * mc.im[q / 3];
+              h = lane_hairer<T, N, CM>(x, v, a0, rtol, atol, max_dt, pk);
+            }
+          }
         } else if (p.geom_status != nullptr && p.geom_status[sg.start + loc] != 0) {
           status = kSimSamplerFail;
           e0 = T(0);
@@ -980,33 +992,34 @@ __global__ void __launch_bounds__(256) sample_sphere(const SampleArgs a) {
 }
 
 // ------------------------------------------------- reference-layout kernels ----
-// batched::accelerations / potential_energy (batched_force.hpp:31-110) on the
-// reference's SoA layout p[i*k + lane], one thread per lane, with the
-// reference's IEEE 1/sqrt and division so results match it bit for bit.
-template <int N>
-__global__ void __launch_bounds__(256) soa_accel(uint64_t k, const float* px, const float* py,
-                                                 const float* pz, const float* pc, const float* im,
-                                                 float* ax, float* ay, float* az) {
+// batched::accelerations<T> / potential_energy<T> (batched_force.hpp:31-110) on
+// the reference's SoA layout p[i*k + lane], one thread per lane, with the
+// reference's IEEE 1/sqrt and division so results match it bit for bit (T =
+// float); T = double backs the double-precision CoulombForce of the C++ API.
+template <class T, int N>
+__global__ void __launch_bounds__(256) soa_accel(uint64_t k, const T* px, const T* py,
+                                                 const T* pz, const T* pc, const T* im,
+                                                 T* ax, T* ay, T* az) {
   const uint64_t l = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (l >= k) return;
-  float x[N], y[N], z[N], fx[N], fy[N], fz[N];
+  T x[N], y[N], z[N], fx[N], fy[N], fz[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     x[i] = px[i * k + l];
     y[i] = py[i * k + l];
     z[i] = pz[i * k + l];
-    fx[i] = fy[i] = fz[i] = 0.0f;
+    fx[i] = fy[i] = fz[i] = T(0);
   }
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = i + 1; j < N; ++j) {
-      const float rx = x[i] - x[j], ry = y[i] - y[j], rz = z[i] - z[j];
-      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-      const float inv = 1.0f / __builtin_sqrtf(d2);
-      const float inv3 = inv / d2;
-      const float s = pc[i * N + j] * inv3;
-      const float gx = rx * s, gy = ry * s, gz = rz * s;
+      const T rx = x[i] - x[j], ry = y[i] - y[j], rz = z[i] - z[j];
+      const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
+      const T inv = T(1) / sqrtT(d2);
+      const T inv3 = inv / d2;
+      const T s = pc[i * N + j] * inv3;
+      const T gx = rx * s, gy = ry * s, gz = rz * s;
       fx[i] += gx;
       fy[i] += gy;
       fz[i] += gz;
@@ -1022,21 +1035,21 @@ __global__ void __launch_bounds__(256) soa_accel(uint64_t k, const float* px, co
   }
 }
 
-template <int N>
-__global__ void __launch_bounds__(256) soa_pe(uint64_t k, const float* px, const float* py,
-                                              const float* pz, const float* pc, float* pe) {
+template <class T, int N>
+__global__ void __launch_bounds__(256) soa_pe(uint64_t k, const T* px, const T* py,
+                                              const T* pz, const T* pc, T* pe) {
   const uint64_t l = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
   if (l >= k) return;
-  float u = 0.0f;
+  T u = T(0);
 #pragma unroll
   for (int i = 0; i < N; ++i)
 #pragma unroll
     for (int j = i + 1; j < N; ++j) {
-      const float rx = px[i * k + l] - px[j * k + l];
-      const float ry = py[i * k + l] - py[j * k + l];
-      const float rz = pz[i * k + l] - pz[j * k + l];
-      const float d2 = __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
-      u = __builtin_fmaf(pc[i * N + j], 1.0f / __builtin_sqrtf(d2), u);
+      const T rx = px[i * k + l] - px[j * k + l];
+      const T ry = py[i * k + l] - py[j * k + l];
+      const T rz = pz[i * k + l] - pz[j * k + l];
+      const T d2 = fmaT(rx, rx, fmaT(ry, ry, rz * rz));
+      u = fmaT(pc[i * N + j], T(1) / sqrtT(d2), u);
     }
   pe[l] = u;
 }

@@ -431,6 +431,16 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
           T unused;
           pair_force<NA, CM, false>(x, k[0], unused, pk);
           if (steps >= p.max_steps) status = kSimNotConverged;
+          if (!(h > T(0))) {  // a given state without a step size: as a fresh sim
+            if (p.h0_mode == kH0Fixed) {
+              h = static_cast<T>(p.h0);
+            } else {
+              T a0[NA3];
+#pragma unroll
+              for (int q = 0; q < NA3; ++q) a0[q] = k[0][q] * im[q / 3];
+              h = pair_hairer<NA, CM>(x, v, a0, im, rtol, atol, max_dt, pk);
+            }
+          }
         } else if (p.geom_status != nullptr && p.geom_status[gs] != 0) {
           status = kSimSamplerFail;
           e0 = T(0);

@@ -258,6 +258,17 @@ int ce_engine_enable_state(ce_engine* eng, int32_t enable);
  * be NULL); n_sims must equal the loaded sims. */
 int ce_engine_get_state(ce_engine* eng, double* pos, double* vel, double* h, double* e0,
                         size_t n_sims);
+/* Sets the state the next run with cfg->resume = 1 starts from, for sims
+ * [0, n_sims) (a mixed engine: all its sims): positions and velocities
+ * [sim][n][3] in `precision` (CE_PRECISION_*; the run must use the same), per-sim
+ * step size h (NULL or <= 0: the run picks the first step as for a fresh sim:
+ * fixed h0 or the Hairer heuristic) and E0 (the energy the PE rule's threshold
+ * and the redistribution refer to; NULL = 0); step counts and time restart at 0.
+ * Enables the saved state.  No reference counterpart as a C entry point: the C++
+ * Integrator::step (integrator.hpp:28) and run_to_convergence (driver.hpp:68) of
+ * include/coulomb_gpu.hpp advance arbitrary states through it. */
+int ce_engine_set_state(ce_engine* eng, const double* pos, const double* vel, const double* h,
+                        const double* e0, size_t n_sims, int32_t precision);
 /* As ce_engine_run, for the listed sims only (single-species engines); the other
  * sims' results are left as they were. */
 int ce_engine_run_subset(ce_engine* eng, const ce_run_config* cfg, const uint64_t* sims,
@@ -390,6 +401,14 @@ int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const floa
 /* Replaces batched::potential_energy<float> (batched_force.hpp:89-110). */
 int ce_batched_potential_energy_f32(size_t n, size_t k, const float* px, const float* py,
                                     const float* pz, const float* pair_const, float* pe);
+/* The same in double: batched::accelerations<double> / potential_energy<double>
+ * (batched_force.hpp:31-110); the C++ CoulombForce (system.hpp:25-39) of
+ * include/coulomb_gpu.hpp evaluates one configuration through them. */
+int ce_batched_accelerations_f64(size_t n, size_t k, const double* px, const double* py,
+                                 const double* pz, const double* pair_const,
+                                 const double* inv_mass, double* ax, double* ay, double* az);
+int ce_batched_potential_energy_f64(size_t n, size_t k, const double* px, const double* py,
+                                    const double* pz, const double* pair_const, double* pe);
 
 #ifdef __cplusplus
 }

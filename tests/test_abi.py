@@ -146,15 +146,29 @@ def test_product_does_not_import_oracle():
                 assert "oracle" not in text.replace("oracle path", ""), fn
 
 
-def test_cpp_mirror_compiles_and_links(tmp_path):
-    """include/coulomb_gpu.hpp builds a reference-style caller and links the library."""
+@pytest.mark.parametrize("name,std,defines", [("generator_loop", "c++20", []),
+                                               ("strategy_api", "c++20", []),
+                                               ("strategy_api", "c++20", ["-DCOULOMB_SINGLE_PRECISION"])])
+def test_cpp_mirror_compiles_and_links(tmp_path, name, std, defines):
+    """include/coulomb_gpu.hpp builds reference-style callers (the generator's
+    batch loop; the sampler / force / integrator / driver API, also with the
+    reference's single-precision Real) and links the library."""
     import subprocess
-    src = os.path.join(ROOT, "tests", "cpp", "mirror_demo.cpp")
-    exe = str(tmp_path / "mirror_demo")
-    r = subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror",
+    src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+    exe = str(tmp_path / name)
+    r = subprocess.run(["g++", "-std=" + std, "-O1", "-Wall", "-Werror", *defines,
                         "-I" + os.path.join(ROOT, "include"), src, "-o", exe,
                         "-L" + os.path.join(ROOT, "coulomb-explosion_amd"), "-lcoulomb_explosion",
                         "-Wl,-rpath," + os.path.join(ROOT, "coulomb-explosion_amd")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+def test_cpp_mirror_header_is_cxx17(tmp_path):
+    src = tmp_path / "h.cpp"
+    src.write_text('#include "coulomb_gpu.hpp"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
+                        "-I" + os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
 
