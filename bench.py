@@ -161,6 +161,25 @@ def accuracy_subset(ce, kind: str, n_sub: int) -> dict:
         steps64.append(a64)
     per_atom = np.concatenate(per_atom)
     cfg_norm = np.concatenate(cfg_norm)
+    # The same f32 results against the asymptote itself: the FP64 path run without an
+    # early stop to t = 1e10 and without redistribution (examples/verify_subset.py:
+    # 74-90 integrates the reference's oracle to t = 1e10), on the first sims.
+    n_asym = min(n_sub, 20000)
+    wa = Workload(ce, kind, n_asym, rtol=1e-10, atol=1e-14)
+    wa.sample()
+    wa.run(ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, t_max=1e10, momentum_tol=0.0,
+                        h0_mode=ce.H0_FIXED, h0=H0, precision=1, redistribute_energy=False))
+    w32s = Workload(ce, kind, n_asym)
+    w32s.sample()
+    w32s.run(run_config(ce, 0))
+    asym_pa, asym_cn = [], []
+    for (p32, _, _), (pinf, st, _) in zip(w32s.momenta_status(), wa.momenta_status()):
+        asym_pa.append((np.linalg.norm(p32 - pinf, axis=-1) /
+                        np.linalg.norm(pinf, axis=-1)).max(axis=-1))
+        k = pinf.shape[0]
+        asym_cn.append(np.linalg.norm((p32 - pinf).reshape(k, -1), axis=1) /
+                       np.linalg.norm(pinf.reshape(k, -1), axis=1))
+    asym_pa, asym_cn = np.concatenate(asym_pa), np.concatenate(asym_cn)
     return {
         "subset": n_sub,
         "vs": "engine FP64 path, rtol 1e-10 atol 1e-14, same stop rule and geometries",
@@ -171,6 +190,17 @@ def accuracy_subset(ce, kind: str, n_sub: int) -> dict:
         "within_1e-5": float(np.mean(per_atom <= 1e-5)),
         "fp64_mean_attempts": float(np.concatenate(steps64).mean()),
         "fp64_kernel_ms": w64.eng.stats()["kernel_ms"],
+        "config_norm_within_1e-5": float(np.mean(cfg_norm <= 1e-5)),
+        "vs_asymptote": {
+            "subset": n_asym,
+            "vs": "engine FP64 path, rtol 1e-10, no early stop, t = 1e10, no redistribution",
+            "per_atom_rel_max": float(asym_pa.max()),
+            "per_atom_rel_median": float(np.median(asym_pa)),
+            "per_atom_rel_p99": float(np.percentile(asym_pa, 99)),
+            "config_norm_rel_max": float(asym_cn.max()),
+            "config_norm_rel_median": float(np.median(asym_cn)),
+            "per_atom_within_1e-5": float(np.mean(asym_pa <= 1e-5)),
+            "per_atom_within_1e-3": float(np.mean(asym_pa <= 1e-3))},
     }
 
 
@@ -192,7 +222,18 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
     rc, g2 = o.sample(SEED, 5, 40000, 4.0, 0.25)
     cfg2 = ffi.make_config(rtol=1e-4, atol=1e-7, stop_rule=ffi.STOP_PE_FRACTION,
                            pe_stop_fraction=1e-5)
-    fid = o.cpu_baseline([12.011 * amu] + [1.008 * amu] * 4, CH4_CHARGES, g2, cfg2, 1, 3)
+    ref_masses = [12.011 * amu] + [1.008 * amu] * 4
+    fid = o.cpu_baseline(ref_masses, CH4_CHARGES, g2, cfg2, 1, 3)
+    # ... and with the reference's own lane width on that setup (8 f32 lanes, AVX2: the
+    # published ~255k sims/s, docs/benchmarks/0008-ch4-dataset-gen.md:61-66)
+    flags = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = next((ln for ln in f if ln.startswith("flags")), "")
+    except OSError:
+        pass
+    fid8 = (o.cpu_baseline(ref_masses, CH4_CHARGES, g2, cfg2, 1, 3, isa="avx2")
+            if " avx2" in flags and " fma" in flags else None)
     # Lockstep efficiency of the baseline on this workload, as the reference
     # defines it (bench/bench_explosion.cpp:298-336): scalar steps of the sims /
     # (K x shared-dt iterations); scalar steps from the per-lane form on a subset.
@@ -218,6 +259,9 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
         "cpu_model": model,
         "host_threads": os.cpu_count(),
         "single_core_at_reference_setup_sims_per_s": fid["sims"] / fid["median_s"],
+        "single_core_at_reference_setup_lanes": fid["lanes"],
+        "single_core_at_reference_setup_avx2_k8_sims_per_s":
+            fid8["sims"] / fid8["median_s"] if fid8 else None,
     }
 
 
@@ -393,6 +437,9 @@ def main() -> int:
     ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-closure", action="store_true",
+                    help="ch4: skip the learned-closure leg (CH4 closure, 2M training "
+                         "trajectories, 8M sims) reported under the line's 'closure' key")
     ap.add_argument("--refill-threshold", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
@@ -548,6 +595,20 @@ def main() -> int:
         print(f"[bench] accuracy subset {args.accuracy_subset}", file=sys.stderr, flush=True)
         acc = accuracy_subset(ce, args.workload, args.accuracy_subset)
 
+    closure = None
+    if args.workload == "ch4" and not args.no_closure:
+        # The closure half of the metric, on its own workload (CH4 learned closure: the
+        # network trained on 2M engine trajectories, then 8M fresh sims per step timed
+        # beside the plain method on the same sims); the line's value stays the plain
+        # method on the headline workload.
+        cargs = argparse.Namespace(**vars(args))
+        cargs.workload, cargs.steps, cargs.warmup = "closure", 3, 1
+        r = run_closure(ce, cargs)
+        closure = dict(r["line_config"])
+        closure["sims_per_s_within_1e-5"] = r["within"] / r["closure_s"]
+        closure["steps"], closure["warmup"] = cargs.steps, cargs.warmup
+        closure["roofline"] = r["roofline"]
+
     cpu = None
     if not args.no_cpu and args.workload == "ch4":
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -610,6 +671,7 @@ def main() -> int:
             "flops_per_attempt": {str(n): flops_per_attempt(n) for n, _ in w.sizes},
         },
         "accuracy": acc,
+        "closure": closure,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -275,18 +275,24 @@ class Oracle:
         fn = self.lib.ceo_batched_run_f32 if precision == "f32" else self.lib.ceo_batched_run_f64
         return self._run(fn, masses, charges, pos, cfg, k, lanes=lanes)
 
-    def cpu_baseline(self, masses, charges, pos, cfg, n_threads, reps, k=1.0):
+    def cpu_baseline(self, masses, charges, pos, cfg, n_threads, reps, k=1.0, isa=None):
+        """isa None: the widest build the host runs; "avx512", "avx2" or "sse2" to
+        pick one (the caller checks that the host supports it)."""
         mh = _MolHold(masses, charges, k)
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         ns = pos.size // (3 * len(mh.m))
         med = C.c_double(0)
         bs = C.c_uint64(0)
         cs = C.c_double(0)
-        lanes = self.lib.ceo_cpu_baseline(C.byref(mh.s), C.byref(cfg), pos.ctypes.data, ns,
-                                          n_threads, reps, C.byref(med), C.byref(bs), C.byref(cs))
+        fn = self.lib.ceo_cpu_baseline if isa is None else getattr(self.lib,
+                                                                    "ceo_cpu_baseline_" + isa)
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_int32, C.c_int32,
+                       C.c_void_p, C.c_void_p, C.c_void_p]
+        lanes = fn(C.byref(mh.s), C.byref(cfg), pos.ctypes.data, ns, n_threads, reps,
+                   C.byref(med), C.byref(bs), C.byref(cs))
         return {"lanes": lanes, "median_s": med.value, "batch_steps": bs.value,
                 "checksum": cs.value, "sims": (ns // max(lanes, 1)) * max(lanes, 1),
-                "isa": self.lib.ceo_cpu_baseline_isa().decode()}
+                "isa": isa or self.lib.ceo_cpu_baseline_isa().decode()}
 
 
 class RefEngine:
