@@ -265,6 +265,37 @@ def cpu_baseline(n_sims: int, reps: int, threads: int) -> dict:
     }
 
 
+def generator_end_to_end(args) -> dict | None:
+    """The dataset generator CLI (coulomb-explosion_amd/generate_dataset) on the
+    headline workload, 60M sims in batches of --sims, records to /dev/null: the
+    rate with every record in (pinned) host memory, copies and the writer thread
+    overlapped with the next batch's integration (examples/generate_dataset.cpp:
+    190-227 times the same loop with its file writes)."""
+    import subprocess
+    exe = os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset")
+    if not os.path.exists(exe):
+        return None
+    total = 6 * args.sims
+    cmd = [exe, "--out", "/dev/null", "--sims", str(total), "--batch", str(args.sims),
+           "--masses", ",".join(str(m) for m in CH4_MASSES), "--charges", "1,1,1,1,1",
+           "--radius", str(RADIUS), "--min-separation", str(MIN_SEP), "--rtol", str(RTOL),
+           "--atol", str(ATOL), "--stop-rule", "tmax", "--t-max", str(T_MAX),
+           "--momentum-tol", str(MOM_TOL), "--h0", str(H0), "--seed", str(SEED),
+           "--progress-every", "1000"]
+    print(f"[bench] generator: {total} sims to /dev/null", file=sys.stderr, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    out = {"sims": total, "batch": args.sims, "command": " ".join(os.path.relpath(c, ROOT)
+                                                                 if c == exe else c for c in cmd)}
+    for ln in r.stdout.splitlines():
+        if ln.startswith("done:"):
+            out["all_sims_per_s"] = float(ln.split("(")[1].split()[0])
+        if ln.startswith("steady:"):
+            out["steady_sims_per_s"] = float(ln.split("(")[1].split()[0])
+    return out
+
+
 CLOSURE_SETUPS = {
     # masses, charge levels (0 = all +1), R, min sep, hidden widths, train / eval sims
     "closure": dict(masses=CH4_MASSES, levels=0, radius=RADIUS, min_sep=MIN_SEP,
@@ -450,8 +481,9 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="tune the launch shape on this workload first (cached)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="only run a plain device copy of this many MiB (HBM counter "
-                         "calibration for rocprofv3 --pmc) and exit")
+                    help="only run the library's plain device copy kernel (hbm_copy) over "
+                         "this many MiB, 3 times (HBM counter calibration for rocprofv3 "
+                         "--pmc), and exit")
     ap.add_argument("--mlp-probe", default="",
                     help="only run the fused inference kernel: widths (e.g. 40,256,256,256,15), "
                          "--sims rows of random inputs, 3 forward passes (counter runs), and exit")
@@ -473,15 +505,15 @@ def main() -> int:
         return 0
 
     if args.copy_probe_mb:
-        import torch
-        n = args.copy_probe_mb * (1 << 20) // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4,
-                          "launches": 3}))
+        import ctypes
+        from __graft_entry__ import load_package
+        ce = load_package()
+        nbytes = args.copy_probe_mb * (1 << 20)
+        ms = ctypes.c_double(0)
+        ce._check(ce.library().ce_hbm_copy_calibration(nbytes, 3, ctypes.byref(ms)))
+        print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                          "launches": 3, "ms_per_copy": ms.value,
+                          "gbs": 2 * nbytes / (ms.value * 1e-3) / 1e9}))
         return 0
 
     if args.gpus != 1:
@@ -574,7 +606,19 @@ def main() -> int:
                      "pack_hbm_gbs": n_rec * (185 + rec.itemsize) / (pack_ms * 1e-3) / 1e9,
                      "host_copy_gbs_incl_pcie": n_rec * rec.itemsize / max(wall - pack_ms * 1e-3,
                                                                           1e-9) / 1e9}
+        # the same records queued into pinned host memory (the generator's path)
+        pinned = ce.PinnedBuffer(n_rec * rec.itemsize)
         del rec
+        w.eng.records_async(pinned)
+        w.eng.records_wait()
+        t1 = time.perf_counter()
+        w.eng.records_async(pinned)
+        w.eng.records_wait()
+        wall = time.perf_counter() - t1
+        writeback["pinned_queued_pack_and_copy_ms"] = wall * 1e3
+        writeback["pinned_host_copy_gbs_incl_pcie"] = n_rec * 125 / wall / 1e9
+        del pinned
+        writeback["generator"] = generator_end_to_end(args)
 
     other = None
     if args.workload == "mixed":

@@ -165,6 +165,12 @@ def _load_library(path: str = LIB_PATH) -> C.CDLL:
         "ce_closure_run": ([P, P, P, C.c_double, C.c_double, P], C.c_int),
         "ce_closure_get_stats": ([P, P], C.c_int),
         "ce_run_device_noalloc": ([P, P, P, P, S, P, P], C.c_int),
+        "ce_hbm_copy_calibration": ([S, C.c_int32, P], C.c_int),
+        "ce_engine_pack_records_async": ([P, C.c_uint64, S, P], C.c_int),
+        "ce_engine_records_wait": ([P], C.c_int),
+        "ce_host_alloc": ([S, P], C.c_int),
+        "ce_host_free": ([P], C.c_int),
+        "ce_engine_set_state": ([P, P, P, P, P, S, C.c_int32], C.c_int),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -496,6 +502,20 @@ class BatchedRK45:
         _check(_lib.ce_engine_pack_records(self._h, first, n_sims, buf.ctypes.data))
         return buf.view(record_dtype(self.n))
 
+    def records_async(self, out: "PinnedBuffer", first: int = 0,
+                      n_sims: int | None = None) -> np.ndarray:
+        """ce_engine_pack_records_async into the pinned buffer `out`: returns its
+        record view, valid after records_wait()."""
+        n_sims = self._loaded - first if n_sims is None else n_sims
+        rb = _lib.ce_record_bytes(self.n)
+        if out.nbytes < n_sims * rb:
+            raise ValueError("pinned buffer too small for the records")
+        _check(_lib.ce_engine_pack_records_async(self._h, first, n_sims, out.ptr))
+        return out.array[: n_sims * rb].view(record_dtype(self.n))
+
+    def records_wait(self) -> None:
+        _check(_lib.ce_engine_records_wait(self._h))
+
     def stats(self) -> dict:
         s = _RunStats()
         _check(_lib.ce_engine_stats(self._h, C.byref(s)))
@@ -665,6 +685,22 @@ class MixedBatch:
                         "t_final": sl[3],
                         "energy_err": self._split_sims(ee)[i] if ee is not None else None})
         return out
+
+
+class PinnedBuffer:
+    """Pinned host memory (ce_host_alloc) with a uint8 numpy view."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        _check(_lib.ce_host_alloc(int(nbytes), C.byref(p)))
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+        self.array = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint8)), shape=(self.nbytes,))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            _lib.ce_host_free(self.ptr)
+            self.ptr = None
 
 
 def record_dtype(n_atoms: int) -> np.dtype:

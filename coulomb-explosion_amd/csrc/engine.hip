@@ -71,6 +71,13 @@ void free_engine(ce_engine* e) {
                   e->records, e->state_x, e->state_v, e->state_h, e->state_e0, e->list};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
+  if (e->copy_stream) (void)hipStreamSynchronize(e->copy_stream);
+  for (int s = 0; s < 2; ++s) {
+    if (e->rec_async[s]) (void)hipFree(e->rec_async[s]);
+    if (e->rec_packed[s]) (void)hipEventDestroy(e->rec_packed[s]);
+    if (e->rec_copied[s]) (void)hipEventDestroy(e->rec_copied[s]);
+  }
+  if (e->copy_stream) (void)hipStreamDestroy(e->copy_stream);
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
   if (e->graph) (void)hipGraphDestroy(e->graph);
   if (e->first_dev) (void)hipFree(e->first_dev);
@@ -1069,6 +1076,82 @@ int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t*
   return CE_OK;
 }
 
+int ce_engine_pack_records_async(ce_engine* e, uint64_t first, size_t n_sims, uint8_t* host_out) {
+  if (e == nullptr || (host_out == nullptr && n_sims > 0))
+    return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (e->n_species != 1)
+    return fail(CE_ERR_INVALID_ARGUMENT, "records are written per molecule: single-species engines only");
+  if (!e->have_results) return fail(CE_ERR_RUNTIME, "no results: run the engine first");
+  if (first > e->loaded() || n_sims > e->loaded() - first)
+    return fail(CE_ERR_INVALID_ARGUMENT, "record range exceeds the loaded sims");
+  if (n_sims == 0) return CE_OK;
+  CE_HIP(hipSetDevice(e->device));
+  if (e->copy_stream == nullptr) {
+    CE_HIP(hipStreamCreateWithFlags(&e->copy_stream, hipStreamNonBlocking));
+    for (int s = 0; s < 2; ++s) {
+      CE_HIP(hipEventCreateWithFlags(&e->rec_packed[s], hipEventDisableTiming));
+      CE_HIP(hipEventCreateWithFlags(&e->rec_copied[s], hipEventDisableTiming));
+    }
+  }
+  const int n = e->sp[0].n;
+  const size_t rb = ce_record_bytes(n);
+  const int s = e->rec_slot;
+  if (e->rec_async[s] == nullptr || e->rec_async_cap[s] < n_sims) {
+    CE_HIP(hipStreamSynchronize(e->copy_stream));  // the buffer's last copy is done
+    if (e->rec_async[s]) CE_HIP(hipFree(e->rec_async[s]));
+    e->rec_async[s] = nullptr;
+    e->rec_async_cap[s] = 0;
+    const int rc = hip_alloc(&e->rec_async[s], n_sims * rb);
+    if (rc != CE_OK) return rc;
+    e->rec_async_cap[s] = n_sims;
+  }
+  // the pack may overwrite this buffer only once its previous copy has left
+  CE_HIP(hipStreamWaitEvent(e->stream, e->rec_copied[s], 0));
+  const dim3 grid(static_cast<unsigned>((n_sims + ce::kRecBlock - 1) / ce::kRecBlock));
+  const dim3 block(ce::kRecBlock);
+  switch (n) {
+#define CE_PACK(NN)                                                                            \
+  case NN:                                                                                     \
+    hipLaunchKernelGGL(ce::pack_records<NN>, grid, block, 0, e->stream, e->geom, e->mom32,     \
+                       e->steps, e->status, first, static_cast<uint64_t>(n_sims),              \
+                       e->rec_async[s]);                                                       \
+    break;
+    CE_PACK(1) CE_PACK(2) CE_PACK(3) CE_PACK(4) CE_PACK(5) CE_PACK(6) CE_PACK(7) CE_PACK(8)
+    CE_PACK(9) CE_PACK(10)
+#undef CE_PACK
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CE_HIP(hipGetLastError());
+  CE_HIP(hipEventRecord(e->rec_packed[s], e->stream));
+  CE_HIP(hipStreamWaitEvent(e->copy_stream, e->rec_packed[s], 0));
+  CE_HIP(hipMemcpyAsync(host_out, e->rec_async[s], n_sims * rb, hipMemcpyDeviceToHost,
+                        e->copy_stream));
+  CE_HIP(hipEventRecord(e->rec_copied[s], e->copy_stream));
+  e->rec_slot = 1 - s;
+  return CE_OK;
+}
+
+int ce_engine_records_wait(ce_engine* e) {
+  if (e == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "engine is NULL");
+  if (e->copy_stream == nullptr) return CE_OK;
+  CE_HIP(hipSetDevice(e->device));
+  CE_HIP(hipStreamSynchronize(e->copy_stream));
+  return CE_OK;
+}
+
+int ce_host_alloc(size_t bytes, void** out) {
+  if (out == nullptr) return fail(CE_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  if (bytes == 0) return CE_OK;
+  CE_HIP(hipHostMalloc(out, bytes, hipHostMallocDefault));
+  return CE_OK;
+}
+
+int ce_host_free(void* p) {
+  if (p != nullptr) CE_HIP(hipHostFree(p));
+  return CE_OK;
+}
+
 int ce_batched_accelerations_f32(size_t n, size_t k, const float* px, const float* py,
                                  const float* pz, const float* pair_const, const float* inv_mass,
                                  float* ax, float* ay, float* az) {
@@ -1198,4 +1281,55 @@ extern "C" int ce_run_device_noalloc(const ce_molecule_desc* mol, const ce_toler
   if (bad_n) return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
   if (he != hipSuccess) return fail(CE_ERR_HIP, std::string("dp5 launch: ") + hipGetErrorString(he));
   return CE_OK;
+}
+
+// ---- ce_hbm_copy_calibration -------------------------------------------------
+namespace {
+
+// A plain device copy: 16-byte loads and stores, grid-stride.
+__global__ void __launch_bounds__(256) hbm_copy(const uint4* __restrict__ src,
+                                               uint4* __restrict__ dst, size_t n16) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < n16; i += stride)
+    dst[i] = src[i];
+}
+
+}  // namespace
+
+extern "C" int ce_hbm_copy_calibration(size_t bytes, int32_t reps, double* ms_per_copy) {
+  if (bytes == 0 || bytes % 16 != 0 || reps < 1 || ms_per_copy == nullptr)
+    return fail(CE_ERR_INVALID_ARGUMENT, "bytes must be a positive multiple of 16, reps >= 1");
+  int dev = 0, cus = 0;
+  CE_HIP(hipGetDevice(&dev));
+  CE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  void *src = nullptr, *dst = nullptr;
+  CE_HIP(hipMalloc(&src, bytes));
+  int rc = CE_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto chk = [&](hipError_t h) {
+    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+  };
+  chk(hipMalloc(&dst, bytes));
+  chk(hipMemset(src, 1, bytes));
+  chk(hipEventCreate(&e0));
+  chk(hipEventCreate(&e1));
+  if (rc == CE_OK) {
+    const size_t n16 = bytes / 16;
+    const unsigned grid = static_cast<unsigned>(std::min<size_t>((n16 + 255) / 256, 32u * cus));
+    chk(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(hbm_copy, dim3(grid), dim3(256), 0, nullptr,
+                         static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+    chk(hipGetLastError());
+    chk(hipEventRecord(e1, nullptr));
+    chk(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    chk(hipEventElapsedTime(&ms, e0, e1));
+    *ms_per_copy = static_cast<double>(ms) / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(dst);
+  (void)hipFree(src);
+  return rc;
 }

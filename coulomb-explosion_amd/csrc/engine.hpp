@@ -61,6 +61,13 @@ struct ce_engine {
   bool eerr_valid = false;   // the last run computed eerr
   uint8_t* records = nullptr;
   size_t records_cap = 0;
+  // queued record copies (ce_engine_pack_records_async): two device buffers
+  // used in turn, packed on `stream`, copied to the host on `copy_stream`
+  hipStream_t copy_stream = nullptr;
+  uint8_t* rec_async[2] = {nullptr, nullptr};
+  size_t rec_async_cap[2] = {0, 0};
+  hipEvent_t rec_packed[2] = {nullptr, nullptr}, rec_copied[2] = {nullptr, nullptr};
+  int rec_slot = 0;
   // integration state at the stop (ce_engine_enable_state): positions and
   // velocities in the run's precision, next step size, E0
   void* state_x = nullptr;

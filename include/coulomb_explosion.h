@@ -310,6 +310,21 @@ size_t ce_record_bytes(int32_t n_atoms);
 /* Packs sims [first, first + n_sims) of the last run into records on the device
  * and copies them to host_out (n_sims * ce_record_bytes(n) bytes). */
 int ce_engine_pack_records(ce_engine* eng, uint64_t first, size_t n_sims, uint8_t* host_out);
+/* The same, queued: the pack kernel runs on the engine's stream and the copy to
+ * host_out on a copy stream of its own, so it overlaps the engine's next run
+ * (the reference generator writes each batch's records before sampling the
+ * next, examples/generate_dataset.cpp:190-217).  Two device record buffers are
+ * used in turn: a third call waits on the device for the first one's copy.
+ * host_out should be pinned host memory (ce_host_alloc) and must not be read,
+ * written or freed before ce_engine_records_wait returns. */
+int ce_engine_pack_records_async(ce_engine* eng, uint64_t first, size_t n_sims,
+                                 uint8_t* host_out);
+/* Waits until every queued record copy has reached host memory. */
+int ce_engine_records_wait(ce_engine* eng);
+/* Pinned (page-locked) host memory, for buffers the device copies into
+ * asynchronously and at the link's full rate; ce_host_free releases it. */
+int ce_host_alloc(size_t bytes, void** out);
+int ce_host_free(void* p);
 
 /* ---- closure network (no reference counterpart) --------------------------
  * A multilayer perceptron: widths[0] inputs, SiLU hidden layers, widths[n-1]
@@ -409,6 +424,13 @@ int ce_batched_accelerations_f64(size_t n, size_t k, const double* px, const dou
                                  const double* inv_mass, double* ax, double* ay, double* az);
 int ce_batched_potential_energy_f64(size_t n, size_t k, const double* px, const double* py,
                                     const double* pz, const double* pair_const, double* pe);
+
+/* No reference counterpart: a plain device-to-device copy of `bytes` (a
+ * multiple of 16) repeated `reps` times on the null stream, average time per
+ * copy into *ms_per_copy.  The reference kernel against which the HBM traffic
+ * counters (rocprofv3 FETCH_SIZE / WRITE_SIZE) of the engine's kernels are
+ * scaled: a copy reads and writes `bytes` each. */
+int ce_hbm_copy_calibration(size_t bytes, int32_t reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

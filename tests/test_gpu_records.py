@@ -38,6 +38,39 @@ def test_records_match_results(gpu, oracle):
     assert raw[124] == rec["converged"][0]
 
 
+def test_queued_records_into_pinned_memory(gpu):
+    """ce_engine_pack_records_async: the synchronous pack's bytes, with two copies
+    in flight (the device buffers alternate) and a third call waiting on the
+    first on the device, and across runs (a copy still in flight while the next
+    run overwrites the results it was packed from)."""
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] +
+                      [ce.Atom("H", m, 1.0) for m in CH4_MASSES[1:]])
+    ns = 3000
+    eng = ce.BatchedRK45(mol, 1.0, ns, 1e-6, 1e-9)
+    cfg = ce.RunConfig(stop_rule=ce.STOP_MOMENTUM_OR_TMAX, h0_mode=ce.H0_FIXED)
+    bufs = [ce.PinnedBuffer(ns * 125) for _ in range(3)]
+    eng.sample(seed=4, n_sims=ns, radius=2.5, min_separation=1.0)
+    eng.run(cfg, download=False)
+    want = eng.records().copy()
+    views = [eng.records_async(bufs[0]), eng.records_async(bufs[1], first=1000, n_sims=1500),
+             eng.records_async(bufs[2], first=17, n_sims=1)]
+    eng.records_wait()
+    assert views[0].tobytes() == want.tobytes()
+    assert views[1].tobytes() == want[1000:2500].tobytes()
+    assert views[2].tobytes() == want[17:18].tobytes()
+    # batch 2 runs while batch 1's copy may still be in flight
+    v1 = eng.records_async(bufs[0])
+    eng.sample(seed=5, n_sims=ns, radius=2.5, min_separation=1.0)
+    eng.run(cfg, download=False)
+    v2 = eng.records_async(bufs[1])
+    eng.records_wait()
+    assert v1.tobytes() == want.tobytes()
+    assert v2.tobytes() == eng.records().tobytes() and v2.tobytes() != want.tobytes()
+    with pytest.raises(ValueError):
+        eng.records_async(ce.PinnedBuffer(100))
+
+
 @pytest.mark.parametrize("graph", [True, False])
 def test_generator_cli(gpu, tmp_path, graph):
     """Two full batches (replayed from a captured graph unless --no-graph) and a
