@@ -261,8 +261,10 @@ int fail(int code, const std::string& m) { return ce::set_error(code, m); }
 #define CL_HIP(call)                                                              \
   do {                                                                            \
     hipError_t e_ = (call);                                                       \
-    if (e_ != hipSuccess)                                                         \
+    if (e_ != hipSuccess) {                                                       \
+      ce::clear_stale_error(); /* reported here: not left for the next check */   \
       return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                             \
   } while (0)
 
 template <class T>
@@ -302,6 +304,7 @@ int ensure_chunk(ce_closure* c, int64_t n) {
 unsigned blocks(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
 
 int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
+  ce::clear_stale_error();
   switch (c->n) {
 #define CL_F(NN)                                                                      \
   case NN:                                                                            \
@@ -319,6 +322,7 @@ int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
 }
 
 int launch_gate(ce_closure* c, const GateArgs& a, hipStream_t s) {
+  ce::clear_stale_error();
   switch (c->n) {
 #define CL_G(NN)                                                                          \
   case NN:                                                                                \
@@ -495,6 +499,7 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
   st.cut_attempts = e->stats.attempted_steps;
   // 2. features, ensemble and gate, chunk by chunk
   CL_HIP(hipEventRecord(c->ev[1], s));
+  ce::clear_stale_error();  // zero_counts is checked with the feature launch
   hipLaunchKernelGGL(zero_counts, dim3(1), dim3(64), 0, s, c->counts);
   const int M = static_cast<int>(c->members.size());
   const int64_t chunk = 1 << 20;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "coulomb_explosion.h"
+#include "hip_check.hpp"
 #include "kernels.hpp"
 
 namespace ce {
@@ -80,7 +81,10 @@ hipError_t launch_one(const LaunchReq& req, int s, LaunchInfo* info) {
   if (per_cu <= 0) {
     int occ = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+      ce::clear_stale_error();
+      return e;
+    }
     per_cu = occ > 0 ? occ : 1;
   }
   RunArgs a = req.args;
@@ -88,6 +92,7 @@ hipError_t launch_one(const LaunchReq& req, int s, LaunchInfo* info) {
   a.n_segments = 1;
   a.queue_slot = s;
   const int grid = per_cu * req.n_cus;
+  ce::clear_stale_error();
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, req.stream, a,
                      make_mol_const<T, N>(req.species[s], req.coulomb_k, CM));
   info->grid = grid;
@@ -123,7 +128,10 @@ hipError_t launch_mixed(const LaunchReq& req, LaunchInfo* info) {
   if (per_cu <= 0) {
     int occ = 0;
     const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kern, block, 0);
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess) {
+      ce::clear_stale_error();
+      return e;
+    }
     per_cu = occ > 0 ? occ : 1;
   }
   MC mc{make_mol_const<T, 2>(req.species[0], req.coulomb_k, CM),
@@ -131,6 +139,7 @@ hipError_t launch_mixed(const LaunchReq& req, LaunchInfo* info) {
         make_mol_const<T, 5>(req.species[2], req.coulomb_k, CM),
         make_mol_const<T, 8>(req.species[3], req.coulomb_k, CM)};
   const int grid = per_cu * req.n_cus;
+  ce::clear_stale_error();
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), 0, req.stream, req.args, mc);
   info->grid = grid;
   info->block = block;

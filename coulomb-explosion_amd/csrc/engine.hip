@@ -33,8 +33,10 @@ int fail(int code, const std::string& msg) {
 #define CE_HIP(call)                                                              \
   do {                                                                            \
     hipError_t e_ = (call);                                                       \
-    if (e_ != hipSuccess)                                                         \
+    if (e_ != hipSuccess) {                                                       \
+      ce::clear_stale_error(); /* reported here: not left for the next check */   \
       return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                             \
   } while (0)
 
 template <class T>
@@ -260,7 +262,9 @@ int soa_accelerations(size_t n, size_t k, const T* px, const T* py, const T* pz,
     *daz = d + 5 * nk, *dpc = d + 6 * nk, *dim = dpc + n * n;
   int rc = CE_OK;
   auto chk = [&](hipError_t h) {
-    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+    if (h == hipSuccess) return;
+    ce::clear_stale_error();
+    if (rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
   };
   chk(hipMemcpy(dx, px, nk * sizeof(T), hipMemcpyHostToDevice));
   chk(hipMemcpy(dy, py, nk * sizeof(T), hipMemcpyHostToDevice));
@@ -269,6 +273,7 @@ int soa_accelerations(size_t n, size_t k, const T* px, const T* py, const T* pz,
   chk(hipMemcpy(dim, inv_mass, n * sizeof(T), hipMemcpyHostToDevice));
   if (rc == CE_OK) {
     const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
+    ce::clear_stale_error();
     switch (n) {
 #define CE_ACC(NN)                                                                           \
   case NN:                                                                                   \
@@ -302,7 +307,9 @@ int soa_potential_energy(size_t n, size_t k, const T* px, const T* py, const T* 
   T *dx = d, *dy = d + nk, *dz = d + 2 * nk, *dpc = d + 3 * nk, *dpe = dpc + n * n;
   int rc = CE_OK;
   auto chk = [&](hipError_t h) {
-    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+    if (h == hipSuccess) return;
+    ce::clear_stale_error();
+    if (rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
   };
   chk(hipMemcpy(dx, px, nk * sizeof(T), hipMemcpyHostToDevice));
   chk(hipMemcpy(dy, py, nk * sizeof(T), hipMemcpyHostToDevice));
@@ -310,6 +317,7 @@ int soa_potential_energy(size_t n, size_t k, const T* px, const T* py, const T* 
   chk(hipMemcpy(dpc, pair_const, n * n * sizeof(T), hipMemcpyHostToDevice));
   if (rc == CE_OK) {
     const dim3 grid(static_cast<unsigned>((k + 255) / 256)), block(256);
+    ce::clear_stale_error();
     switch (n) {
 #define CE_PE(NN)                                                                               \
   case NN:                                                                                      \
@@ -357,6 +365,7 @@ int sample_segment(ce_engine* e, int s, uint64_t seed, uint64_t first_global, do
   a.charge_levels = e->sp[s].levels;
   a.max_total_charge = e->sp[s].max_total;
   if (a.n_sims == 0) return CE_OK;
+  ce::clear_stale_error();
   switch (e->sp[s].n) {
     case 1: launch_sample<1>(a, e->stream); break;
     case 2: launch_sample<2>(a, e->stream); break;
@@ -898,6 +907,7 @@ int prepare_run(ce_engine* e, const ce_run_config* cfg, const uint32_t* list, si
 // (no memset / copy nodes), so the sequence can be captured into a graph.
 int launch_run(ce_engine* e, const ce::LaunchReq& req, int precision, size_t n_run,
                ce::LaunchInfo* info) {
+  ce::clear_stale_error();
   hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, e->stream, e->ctr, ce::kCtrWords);
   CE_HIP(hipGetLastError());
   bool bad_n = false;
@@ -1004,6 +1014,7 @@ int ce_engine_replay(ce_engine* e, uint64_t first_index, ce_results* out) {
   e->sp[0].count = e->graph_sims;
   e->have_charges = false;
   e->sampled = true;
+  ce::clear_stale_error();
   hipLaunchKernelGGL(ce::set_u64, dim3(1), dim3(64), 0, e->stream, e->first_dev, first_index);
   CE_HIP(hipGetLastError());
   CE_HIP(hipEventRecord(e->k0, e->stream));
@@ -1055,6 +1066,7 @@ int ce_engine_pack_records(ce_engine* e, uint64_t first, size_t n_sims, uint8_t*
   const dim3 grid(static_cast<unsigned>((n_sims + ce::kRecBlock - 1) / ce::kRecBlock));
   const dim3 block(ce::kRecBlock);
   CE_HIP(hipEventRecord(e->p0, e->stream));
+  ce::clear_stale_error();
   switch (n) {
 #define CE_PACK(NN)                                                                            \
   case NN:                                                                                     \
@@ -1109,6 +1121,7 @@ int ce_engine_pack_records_async(ce_engine* e, uint64_t first, size_t n_sims, ui
   CE_HIP(hipStreamWaitEvent(e->stream, e->rec_copied[s], 0));
   const dim3 grid(static_cast<unsigned>((n_sims + ce::kRecBlock - 1) / ce::kRecBlock));
   const dim3 block(ce::kRecBlock);
+  ce::clear_stale_error();
   switch (n) {
 #define CE_PACK(NN)                                                                            \
   case NN:                                                                                     \
@@ -1272,6 +1285,7 @@ extern "C" int ce_run_device_noalloc(const ce_molecule_desc* mol, const ce_toler
   req.n_cus = dev.n_cus;
   req.stream = static_cast<hipStream_t>(stream);
   // The counters start at zero on every call (a kernel, not a memset node).
+  ce::clear_stale_error();
   hipLaunchKernelGGL(ce::zero_u64, dim3(1), dim3(64), 0, req.stream, dev.scratch, ce::kCtrWords);
   CE_HIP(hipGetLastError());
   ce::LaunchInfo info;
@@ -1307,7 +1321,9 @@ extern "C" int ce_hbm_copy_calibration(size_t bytes, int32_t reps, double* ms_pe
   int rc = CE_OK;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   auto chk = [&](hipError_t h) {
-    if (h != hipSuccess && rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
+    if (h == hipSuccess) return;
+    ce::clear_stale_error();
+    if (rc == CE_OK) rc = fail(CE_ERR_HIP, hipGetErrorString(h));
   };
   chk(hipMalloc(&dst, bytes));
   chk(hipMemset(src, 1, bytes));
@@ -1317,6 +1333,7 @@ extern "C" int ce_hbm_copy_calibration(size_t bytes, int32_t reps, double* ms_pe
     const size_t n16 = bytes / 16;
     const unsigned grid = static_cast<unsigned>(std::min<size_t>((n16 + 255) / 256, 32u * cus));
     chk(hipEventRecord(e0, nullptr));
+    ce::clear_stale_error();
     for (int r = 0; r < reps; ++r)
       hipLaunchKernelGGL(hbm_copy, dim3(grid), dim3(256), 0, nullptr,
                          static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);

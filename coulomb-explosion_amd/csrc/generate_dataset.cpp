@@ -14,23 +14,21 @@
 // b's records while it computes batch b + 1, and a writer thread writes them
 // while it computes batch b + 2.  --out /dev/null measures the pipeline with the
 // records in host memory (no sidecar is written then).
-#include <chrono>
 #include <algorithm>
-#include <condition_variable>
-#include <deque>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
-#include <mutex>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "coulomb_explosion.h"
+#include "record_writer.hpp"
 
 namespace {
 
@@ -163,58 +161,6 @@ void write_metadata(const Args& a, double wall, double rate, std::size_t n_faile
       << "}\n";
 }
 
-// One writer thread writes each batch's records to the file while the device
-// computes the next batch and copies the one after.  Buffers are written in the
-// order they are submitted; wait_written(k) returns once k of them are done.
-struct Writer {
-  std::ofstream out;
-  std::mutex mu;
-  std::condition_variable cv;
-  std::deque<std::pair<const uint8_t*, std::size_t>> queue;
-  std::size_t submitted = 0, written = 0;
-  bool stop = false;
-  std::thread th;
-
-  explicit Writer(const std::string& path) : out(path, std::ios::binary) {
-    if (!out) die("cannot open output file: " + path);
-    th = std::thread([this] {
-      for (;;) {
-        std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [this] { return !queue.empty() || stop; });
-        if (queue.empty()) return;
-        const auto [buf, bytes] = queue.front();
-        lk.unlock();
-        out.write(reinterpret_cast<const char*>(buf), static_cast<std::streamsize>(bytes));
-        lk.lock();
-        queue.pop_front();
-        ++written;
-        cv.notify_all();
-      }
-    });
-  }
-  void submit(const uint8_t* buf, std::size_t bytes) {
-    std::lock_guard<std::mutex> lk(mu);
-    queue.emplace_back(buf, bytes);
-    ++submitted;
-    cv.notify_all();
-  }
-  void wait_written(std::size_t k) {
-    std::unique_lock<std::mutex> lk(mu);
-    cv.wait(lk, [this, k] { return written >= k; });
-  }
-  void finish() {
-    {
-      std::unique_lock<std::mutex> lk(mu);
-      cv.wait(lk, [this] { return queue.empty(); });
-      stop = true;
-      cv.notify_all();
-    }
-    th.join();
-    out.close();
-    if (!out) die("writing the records failed");
-  }
-};
-
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -251,7 +197,13 @@ int main(int argc, char** argv) {
     check(ce_host_alloc(batch * rb, reinterpret_cast<void**>(&h)), "ce_host_alloc");
   std::vector<double> host_pos;
   if (hs) host_pos.resize(batch * 3 * n);
-  Writer writer(a.out);
+  std::unique_ptr<ce::RecordWriter> writer_owner;
+  try {
+    writer_owner = std::make_unique<ce::RecordWriter>(a.out);
+  } catch (const std::exception& e) {
+    die(e.what());
+  }
+  ce::RecordWriter& writer = *writer_owner;
   std::vector<uint8_t> status(batch);
   std::size_t done = 0, b = 0, prev_cnt = 0, n_failed = 0;
   const auto t0 = std::chrono::steady_clock::now();
@@ -299,7 +251,11 @@ int main(int argc, char** argv) {
   }
   check(ce_engine_records_wait(eng), "ce_engine_records_wait");
   writer.submit(host[(b - 1) % 2], prev_cnt * rb);
-  writer.finish();
+  try {
+    writer.finish();
+  } catch (const std::exception& e) {
+    die(e.what());
+  }
   std::fprintf(stderr, "\n");
   const auto t_end = std::chrono::steady_clock::now();
   const double wall = std::chrono::duration<double>(t_end - t0).count();

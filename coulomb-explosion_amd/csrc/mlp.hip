@@ -597,8 +597,10 @@ int fail(int code, const std::string& m) { return ce::set_error(code, m); }
 #define MLP_HIP(call)                                                             \
   do {                                                                            \
     hipError_t e_ = (call);                                                       \
-    if (e_ != hipSuccess)                                                         \
+    if (e_ != hipSuccess) {                                                       \
+      ce::clear_stale_error(); /* reported here: not left for the next check */   \
       return fail(CE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    }                                                                             \
   } while (0)
 
 template <class T>
@@ -682,6 +684,7 @@ int gemm(hipStream_t s, GemmArgs g, bool split_k = false) {
 int round_up(int v, int m) { return (v + m - 1) / m * m; }
 
 int export_all(ce_mlp* m) {
+  ce::clear_stale_error();
   for (int l = 0; l < m->L; ++l) {
     const int in = m->widths[l], out = m->widths[l + 1];
     const int64_t n = static_cast<int64_t>(in) * out;
@@ -916,6 +919,7 @@ int ce_mlp_forward(ce_mlp* m, const float* x_host, size_t batch, float* y_host) 
     if (rc != CE_OK) return rc;
     MLP_HIP(hipMemcpyAsync(m->xin, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
                            hipMemcpyHostToDevice, m->stream));
+    ce::clear_stale_error();
     rc = (m->fused_h && m->use_fused) ? ce::mlp_forward_device(m, m->xin, B, m->out, m->stream)
                                       : forward(m, m->xin, B, false, m->out, m->stream);
     if (rc != CE_OK) return rc;
@@ -939,6 +943,7 @@ int ce_mlp_gradients(ce_mlp* m, const float* x_host, const float* t_host, size_t
   const int wo = m->widths[m->L];
   MLP_HIP(hipMemcpyAsync(m->xin, x_host, B * m->widths[0] * sizeof(float), hipMemcpyHostToDevice, m->stream));
   MLP_HIP(hipMemcpyAsync(m->tgt, t_host, B * wo * sizeof(float), hipMemcpyHostToDevice, m->stream));
+  ce::clear_stale_error();
   rc = forward(m, m->xin, B, true, m->out, m->stream);
   if (rc != CE_OK) return rc;
   MLP_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
@@ -993,6 +998,7 @@ int adam_step(ce_mlp* m, float lr) {
   m->adam_t += 1;
   const float c1 = 1.0f - std::pow(b1, static_cast<float>(m->adam_t));
   const float c2 = 1.0f - std::pow(b2, static_cast<float>(m->adam_t));
+  ce::clear_stale_error();
   hipLaunchKernelGGL(adam, dim3(blocks_for(m->n_params)), dim3(256), 0, m->stream, m->P, m->G,
                      m->M1, m->M2, m->n_params, lr, b1, b2, eps, c1, c2);
   MLP_HIP(hipGetLastError());
@@ -1031,6 +1037,7 @@ int ce_mlp_train_epoch(ce_mlp* m, size_t batch, float lr, uint64_t shuffle_seed,
   MLP_HIP(hipMemsetAsync(m->loss, 0, sizeof(float), m->stream));
   int64_t n_batches = 0;
   for (int64_t b0 = 0; b0 + B <= n; b0 += B, ++n_batches) {  // whole batches only
+    ce::clear_stale_error();  // the batch's launches are checked together, by backward
     hipLaunchKernelGGL(gather_rows, dim3(blocks_for(B * wi)), dim3(256), 0, m->stream, m->ds_x,
                        m->perm + b0, B, wi, m->xin);
     hipLaunchKernelGGL(gather_rows, dim3(blocks_for(B * wo)), dim3(256), 0, m->stream, m->ds_t,
@@ -1078,6 +1085,7 @@ int mlp_width(const ce_mlp* m, int layer) {
 int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 
 int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
+  ce::clear_stale_error();
   if (m->fused_h && m->use_fused && B > 0) {
     if (m->wp_stale) {
       for (int l = 0; l < m->L; ++l) {

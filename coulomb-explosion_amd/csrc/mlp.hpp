@@ -6,6 +6,7 @@
 #include <cstdint>
 
 #include "coulomb_explosion.h"
+#include "hip_check.hpp"
 
 namespace ce {
 

@@ -179,6 +179,33 @@ def test_argument_checks_fail_cleanly(gpu):
     np.testing.assert_array_equal(a.steps, b.steps)
 
 
+def test_a_failed_allocation_does_not_fail_later_launches(gpu):
+    """HIP keeps a thread's last error until it is read: an engine whose device
+    buffers do not fit (2^40 CH4 sims) fails with the HIP error, and the
+    launches of engines made afterwards (device sampler, integrator, records,
+    the mixed-size kernel and its sampler) still succeed, with the same results
+    as before the failure."""
+    ce = gpu
+    mol = ce.Molecule([ce.Atom("C", CH4_MASSES[0], 1.0)] + [ce.Atom("H", CH4_MASSES[1], 1.0)] * 4)
+    eng = ce.BatchedRK45(mol, 1.0, 800, 1e-6, 1e-9)
+    eng.sample(3, 800, 2.5, 1.0)
+    before = eng.run(ce.RunConfig(**RULE))
+    with pytest.raises(RuntimeError, match="hipMalloc"):
+        ce.BatchedRK45(mol, 1.0, 2**40, 1e-6, 1e-9)
+    eng2 = ce.BatchedRK45(mol, 1.0, 800, 1e-6, 1e-9)
+    eng2.sample(3, 800, 2.5, 1.0)
+    after = eng2.run(ce.RunConfig(**RULE))
+    np.testing.assert_array_equal(after.momenta, before.momenta)
+    assert len(eng2.records(0, 800)) == 800
+    with pytest.raises(RuntimeError, match="hipMalloc"):
+        ce.BatchedRK45(mol, 1.0, 2**40, 1e-6, 1e-9)
+    mixed = ce.MixedBatch([ce.Species(mol, 600, 3.2, 0.9, charge_levels=2, max_total_charge=10)],
+                          1.0, 1e-6, 1e-9)
+    mixed.sample(5)
+    (res,) = mixed.run(ce.RunConfig(**RULE))
+    assert (res["status"] <= ce.SIM_OK_TMAX).all()
+
+
 def test_forced_failures_end_sims_with_their_status(gpu, oracle):
     """Failures end a sim with a status, never the run, as the oracle's lane
     statement has them: max_steps reached (not converged, after exactly
