@@ -681,10 +681,11 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         const T rx = (g[q / 3] * ex) * rcpT(scx);
         exs = fmaT(rx, rx, exs);
       }
-      {
-        T unused;
-        lane_force<T, N, CM, false, false>(xs, k[6], unused, pk);
-      }
+      // F(x_new) also gives the potential energy at x_new, which the PE rule
+      // tests after an accepted step: the same pair terms in the same order as
+      // lane_pe, so the same bits, for one FMA per pair.
+      T pe_new;
+      lane_force<T, N, CM, true, false>(xs, k[6], pe_new, pk);
 #pragma unroll
       for (int q = 0; q < N3; ++q) {
         const T ev = fmaT(static_cast<T>(kDP5.e[6]), k[6][q], evp[q]);
@@ -737,8 +738,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         ++tot.acc;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
-          if (static_cast<double>(lane_pe<T, N, CM>(x, pk)) <=
-              p.pe_stop_fraction * static_cast<double>(e0))
+          if (static_cast<double>(pe_new) <= p.pe_stop_fraction * static_cast<double>(e0))
             status = kSimOkPE;
         } else if (final_step) {
           status = kSimOkTmax;

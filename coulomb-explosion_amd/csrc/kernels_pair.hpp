@@ -184,14 +184,6 @@ __device__ __forceinline__ void pair_force(const float (&xo)[3 * NA], float (&a)
   if constexpr (WITH_PE) pe = pk.pe(pair_sum(u));
 }
 
-// Potential energy of the sim (both halves combined).
-template <int NA, int CM>
-__device__ __forceinline__ float pair_pe(const float (&xo)[3 * NA], const PairLaneK<NA, CM>& pk) {
-  float a[3 * NA], pe;
-  pair_force<NA, CM, true>(xo, a, pe, pk);
-  return pe;
-}
-
 // Hairer / scipy initial step (src/integrators.cpp:197-250) over both halves.
 template <int NA, int CM>
 __device__ float pair_hairer(const float (&x)[3 * NA], const float (&v)[3 * NA],
@@ -524,10 +516,10 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         const T rx = (g[q / 3] * ex) * rcpT(scx);
         exs = fmaT(rx, rx, exs);
       }
-      {
-        T unused;
-        pair_force<NA, CM, false>(xs, k[6], unused, pk);
-      }
+      // F(x_new) also gives the potential energy at x_new, which the PE rule
+      // tests after an accepted step
+      T pe_new;
+      pair_force<NA, CM, true>(xs, k[6], pe_new, pk);
 #pragma unroll
       for (int q = 0; q < NA3; ++q) {
         const T ev = fmaT(static_cast<T>(kDP5.e[6]), k[6][q], evp[q]);
@@ -578,8 +570,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         if (lead) ++tot.acc;
         t = final_step ? p.t_max : t + static_cast<double>(hh);
         if (p.stop_rule == kStopPE) {
-          if (static_cast<double>(pair_pe<NA, CM>(x, pk)) <=
-              p.pe_stop_fraction * static_cast<double>(e0))
+          if (static_cast<double>(pe_new) <= p.pe_stop_fraction * static_cast<double>(e0))
             status = kSimOkPE;
         } else if (final_step) {
           status = kSimOkTmax;
