@@ -5,10 +5,15 @@ simulation per lane), the resident blocks per CU (occupancy actually launched),
 the refill threshold (idle lanes a wave waits for before it pulls new
 simulations) and the queue chunk (simulations a wave claims per atomic).  The
 pair loop and the stage sums are fully unrolled per molecule size at compile
-time, so the unroll is fixed by N.  The tuner times the integration kernel (HIP
-events on the engine stream) on the engine's loaded workload, coordinate by
-coordinate, and caches the best shape per (device, workload, precision, stop
-rule) in tuned_launch.json next to this file.
+time, so the unroll inside a step is fixed by N.  The other unroll, of the
+scheduling loop (2 or 4 attempted steps per finalize / refill check), was
+measured on 10M CH4 sims: no faster (14.55 and 14.61 ms against 14.54 at one
+step per check), and its run-time switch alone slowed the one-step kernel from
+13.82 to 14.54 ms (one more scalar register live through the step), so it is
+not a tuned dimension.  The tuner times the integration kernel (HIP events on
+the engine stream) on the engine's loaded workload, coordinate by coordinate,
+never measuring a shape twice, and caches the best shape per (device,
+workload, precision, stop rule) in tuned_launch.json next to this file.
 """
 from __future__ import annotations
 
@@ -67,12 +72,14 @@ def tune(ce, eng, cfg, key_extra: str = "", rounds: int = 1, log=print) -> dict:
     eng.run(cfg, download=False)  # warm
     best_ms = time_shape(eng, cfg, shape)
     trials = [dict(shape, kernel_ms=best_ms)]
+    tried = {tuple(sorted(shape.items()))}
     for _ in range(rounds):
         for name, values in SPACE.items():
             for v in values:
-                if v == shape[name]:
-                    continue
                 cand = dict(shape, **{name: v})
+                if tuple(sorted(cand.items())) in tried:  # measured already
+                    continue
+                tried.add(tuple(sorted(cand.items())))
                 try:
                     ms = time_shape(eng, cfg, cand)
                 except (ValueError, RuntimeError) as e:  # shape rejected by the engine
