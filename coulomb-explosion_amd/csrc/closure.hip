@@ -162,17 +162,34 @@ struct GateArgs {
   uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut
 };
 
-// Wave-aggregated append: one atomic per wave, each flagged lane gets its slot.
-__device__ __forceinline__ void wave_append(bool flag, uint32_t value, uint32_t* counter,
-                                            uint32_t* list) {
-  const uint64_t mask = __ballot(flag);
-  if (mask == 0) return;
-  const int lane = __lane_id();
-  uint32_t base = 0;
-  if (lane == __ffsll(static_cast<long long>(mask)) - 1)
-    base = atomicAdd(counter, static_cast<uint32_t>(__popcll(mask)));
-  base = __shfl(base, __ffsll(static_cast<long long>(mask)) - 1);
-  if (flag) list[base + __popcll(mask & ((1ull << lane) - 1))] = value;
+// Block-aggregated append (256 threads): one atomic per block on each counter
+// (the fallback list and the candidate count), each flagged lane gets its slot.
+// Per-wave atomics on two addresses serialise at the memory side when every sim
+// falls back (two per 64 sims).
+__device__ __forceinline__ void block_append(bool cand, bool flag, uint32_t value, uint32_t* counts,
+                                             uint32_t* list) {
+  __shared__ uint32_t wave_n[4][2];
+  __shared__ uint32_t base_s;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const uint64_t fmask = __ballot(flag), cmask = __ballot(cand);
+  if (lane == 0) {
+    wave_n[wave][0] = static_cast<uint32_t>(__popcll(fmask));
+    wave_n[wave][1] = static_cast<uint32_t>(__popcll(cmask));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t nf = 0, nc = 0;
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t f = wave_n[w][0];
+      wave_n[w][0] = nf;  // exclusive prefix: the wave's offset in the block's slots
+      nf += f;
+      nc += wave_n[w][1];
+    }
+    base_s = nf ? atomicAdd(counts, nf) : 0u;
+    if (nc) atomicAdd(counts + 1, nc);
+  }
+  __syncthreads();
+  if (flag) list[base_s + wave_n[wave][0] + __popcll(fmask & ((1ull << lane) - 1))] = value;
 }
 
 template <int N>
@@ -225,9 +242,7 @@ __global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
       a.status[g] = ce::kSimClosure;
     }
   }
-  const uint64_t cmask = __ballot(cand);
-  if (__lane_id() == 0 && cmask != 0) atomicAdd(a.counts + 1, static_cast<uint32_t>(__popcll(cmask)));
-  wave_append(cand && !ok, static_cast<uint32_t>(g), a.counts, a.list);
+  block_append(cand, cand && !ok, static_cast<uint32_t>(g), a.counts, a.list);
 }
 
 __global__ void zero_counts(uint32_t* p) {

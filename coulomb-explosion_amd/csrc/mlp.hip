@@ -301,153 +301,146 @@ __global__ void adam(float* p, const float* g, float* m, float* v, int64_t n, fl
 
 
 // ---------------------------------------------------------- fused forward ----
-// Inference of a whole network [w0, H, ..., H, w_out] (H = 256 with w0 <= 64, or
-// H = 512 with w0 <= 128; w_out <= 32) in one kernel, computed transposed:
-// C^T[n][m] = sum_k W[n][k] act[m][k], so the weights are the MFMA A operand,
-// read from LDS, and the activations are the B operand, held in registers for
-// the whole network.  A workgroup owns 4 waves x 32 samples; the weights stream
-// through LDS in 32-row output blocks (all K columns), double-buffered and shared
-// by the 4 waves, later tiles' global loads in flight while the current one feeds
-// the MFMAs.  Between layers each lane turns its accumulators
-// (features 8q + 4h + t of a 32-feature block, h = lane / 32) into the next
-// layer's B fragments (8 consecutive features per lane) with one
-// v_permlane32_swap per dword pair.  Products and their K order are those of
-// the layer-by-layer GEMM.
-constexpr int kFusedMaxLayers = 8;
+// Inference of a whole network [w0, H, ..., H, w_out] (H = 256 or 512, w0 + 2 <=
+// 64 or 128, w_out <= 32, 2..kFusedMaxLayers weight layers) in one kernel,
+// computed transposed: C^T[n][m] = sum_k W[n][k] act[m][k], so the weights are the
+// MFMA A operand, streamed through LDS, and the activations the B operand, held in
+// registers from the first layer to the last.  A workgroup owns 4 waves x 32
+// samples; the accumulator layout of a 32-feature block (lane holds features
+// 8q + 4h + t, h = lane / 32) becomes the next layer's operand layout (8
+// consecutive features per lane) with one v_permlane32_swap per dword pair.
+//
+// Weight stream.  All layers' bf16 weights lie back to back ([H][k0], then [H][H]
+// per hidden layer, then [32][H]) and are consumed as equal tiles of 32 H
+// elements: layer 0 as tiles of 32 H / k0 rows (several output blocks each), every
+// later layer as one 32-row output block per tile.  Tile t + 2's global loads
+// (t + 1's at H = 512) are issued when tile t starts and land in registers; tile
+// t + 1 goes from registers to the other LDS buffer while tile t feeds the MFMAs.  Tiles, layers and blocks
+// are unrolled at compile time (the layer count is a template parameter), so every
+// address is a constant offset and every wait count exact.
+//
+// Log2-scaled activations.  Hidden layers carry a = log2(e) SiLU(z) (rounded to
+// bf16) instead of SiLU(z), and their biases are log2(e) b (FP32), so the MFMA sums
+// are z' = log2(e) z and SiLU costs one exp2, one add, one reciprocal and one
+// multiply: a = z' / (1 + 2^-z').  Layer 0 (raw inputs) scales its sums by log2(e)
+// in its epilogue; the output layer's bias enters as log2(e) b_out and its sums are
+// divided by log2(e) at the end.  The weights are the training path's bf16
+// values; results agree with the layer-by-layer path to bf16 rounding of the
+// activations, not bit for bit.
+//
+// Biases.  Hidden and output blocks start their accumulators from the block's
+// biases, loaded before the weight prefetch issued with them (so waiting for them
+// never waits for weights); layer 0's bias is two extra weight columns,
+// the bf16 high and low parts of b, against constant-1 inputs at w0 and w0 + 1.
+//
+// Scheduling.  A block's SiLU epilogue (VALU) is emitted with the next block's
+// MFMA chain, which does not depend on it, so the two interleave.
+constexpr int kFusedMaxLayers = 6;     // H = 256
+constexpr int kFusedMaxLayers512 = 5;  // H = 512 (code size)
+constexpr float kLog2e = 1.4426950408889634f;
 
 struct FusedArgs {
-  const float* x;  // [rows][w0] fp32
+  const float* x;     // [rows][w0] fp32
   int64_t rows;
-  int w0;
-  int layers;      // weight layers (>= 2): w0->H, (layers - 2) x H->H, H->w_out
-  int w_out;
-  // Padded bf16 weights of all layers back to back: [H][k0] (k0 = round64(w0)),
-  // then [H][H] per hidden layer, then [32][H]; layer l >= 1 starts at
-  // W + H k0 + (l - 1) H H (uniform arithmetic, no per-block table lookup).
-  const uint16_t* W;
-  // Per-layer bias pointers, read only through layer_b (constant indices:
-  // indexing a kernel-argument array with a run-time layer would copy the
-  // arguments to scratch).
-  const float* b[kFusedMaxLayers];     // [out]
-  float* y;                            // [rows][w_out]
+  int w0;             // inputs; columns w0 and w0 + 1 of layer 0 hold its bias
+  int w_out;          // outputs (<= 32)
+  const uint16_t* W;  // the weight stream
+  const float* b;     // log2(e) b of layers 1 .. L - 1: layer l at b + (l - 1) H (output: 32)
+  float* y;           // [rows][w_out]
 };
 
-__device__ __forceinline__ const float* layer_b(const FusedArgs& a, int l) {
-  switch (l) {
-    case 0: return a.b[0];
-    case 1: return a.b[1];
-    case 2: return a.b[2];
-    case 3: return a.b[3];
-    case 4: return a.b[4];
-    case 5: return a.b[5];
-    case 6: return a.b[6];
-    default: return a.b[7];
-  }
-}
-
-// Output-block-major fused inference (H = 256 or 512): the weights stream through
-// LDS one 32-row output block (all K columns) at a time, so a block's accumulators
-// are final after its tile and its SiLU epilogue (VALU) overlaps the next block's
-// MFMAs.  Each wave holds 32 samples: the input fragments (act) and the next
-// layer's fragments (nxt) in registers, H / 4 VGPRs each -- two waves per SIMD at
-// H = 256 (their MFMA and VALU work also overlap), one at H = 512.
-// Loads through pointers read from the device pointer table: cast to the global
-// address space, so they are global loads (vmcnt only) rather than flat loads,
-// whose lgkmcnt share would make every LDS wait also wait for the weight prefetch.
+// Global-address loads (vmcnt only): through a generic pointer they would be flat
+// loads, whose lgkmcnt share makes every LDS wait also wait for the prefetch.
 typedef const u32x4 __attribute__((address_space(1))) global_u32x4;
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef const f32x4 __attribute__((address_space(1))) global_f32x4;
-__device__ __forceinline__ u32x4 load_global16(const void* p) {
-  return *(global_u32x4*)(p);
-}
+__device__ __forceinline__ u32x4 load_global16(const void* p) { return *(global_u32x4*)(p); }
 __device__ __forceinline__ f32x4 load_global_f32x4(const float* p) {
   return *(global_f32x4*)(p);
 }
 
-template <int H>
-struct Nout {
-  static constexpr int TS = H + 8;         // LDS tile row stride (elements)
-  static constexpr int NB = H / 32;        // output blocks of a hidden layer
-  static constexpr int KS = H / 16;        // k16 steps of a hidden layer's input
-  static constexpr int LD = 32 * H / 2048; // 16-byte chunks per thread of a full tile
+template <int H, int K0S, int L>
+struct Fz {
+  static constexpr int HID = H;                       // hidden width
+  static constexpr int K0 = 16 * K0S;                 // padded input width
+  static constexpr int KS = H / 16;                   // k16 steps of a later layer
+  static constexpr int NB = H / 32;                   // output blocks of layer 0 / a hidden layer
+  static constexpr int TILE = 32 * H;                 // elements of one streamed tile
+  static constexpr int CH = TILE / (8 * 256);         // 16-byte chunks per thread and tile
+  static constexpr int R0 = TILE / K0;                // rows of a layer-0 tile
+  static constexpr int B0 = R0 / 32;                  // output blocks of a layer-0 tile
+  static constexpr int T0 = H / R0;                   // layer-0 tiles
+  static constexpr int NT = T0 + (L - 2) * NB + 1;    // tiles in the stream
+  static constexpr int NBLK = NB + (L - 2) * NB + 1;  // output blocks
+  static constexpr int S0 = K0 + 8, SH = H + 8;       // LDS row strides (elements)
+  static constexpr int LDS = (R0 * S0 > 32 * SH) ? R0 * S0 : 32 * SH;
+  // Tiles in flight in registers: two ahead at H = 256; one at H = 512, whose
+  // activations (two 128-register sets) leave no room for a second.
+  static constexpr int PD = H == 256 ? 2 : 1;
+  static_assert(K0 % 32 == 0 && B0 * K0S == KS && T0 * B0 == NB, "tile geometry");
+  static constexpr int tile_of(int g) { return g < NB ? g / B0 : T0 + (g - NB); }
+  static constexpr int pos_of(int g) { return g < NB ? g % B0 : 0; }
+  static constexpr int first_of(int t) { return t < T0 ? t * B0 : NB + (t - T0); }
+  static constexpr int blocks_of(int t) { return t < T0 ? B0 : 1; }
+  static constexpr int layer_of(int g) { return g < NB ? 0 : 1 + (g - NB) / NB; }
+  static constexpr int nb_of(int g) { return g < NB ? g : (g - NB) % NB; }
 };
 
-// Tile t of the stream: layer 0's NB blocks (k0 columns), the hidden layers'
-// blocks, then the output layer's single block.
-template <int H>
-__device__ __forceinline__ void nout_tile(const FusedArgs& a, int t, int k0, const uint16_t** W,
-                                          int* ldw, int* row0) {
-  constexpr int NB = Nout<H>::NB;
-  if (t < NB) {
-    *W = a.W;
-    *ldw = k0;
-    *row0 = 32 * t;
-  } else {
-    const int l = 1 + (t - NB) / NB;
-    *W = a.W + static_cast<int64_t>(H) * k0 + static_cast<int64_t>(l - 1) * H * H;
-    *ldw = H;
-    *row0 = l == a.layers - 1 ? 0 : 32 * ((t - NB) % NB);
+template <int P, class A>
+__device__ __forceinline__ A& pick(A& a0, A& a1) {
+  if constexpr (P == 0)
+    return a0;
+  else
+    return a1;
+}
+
+// Tile TL's weights into registers (nothing past the last tile).
+template <class F, int TL>
+__device__ __forceinline__ void fz_fetch(const uint16_t* W, u32x4 (&r)[F::CH]) {
+  if constexpr (TL < F::NT) {
+    const uint16_t* p = W + static_cast<int64_t>(TL) * F::TILE + 8 * threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < F::CH; ++c) r[c] = load_global16(p + 2048 * c);
   }
 }
 
-// Tile t's weights into registers; past the last tile the last one is loaded
-// again, so every block issues the same loads and the compiler's wait counts stay
-// exact (a load issued on only some paths makes later waits conservative).
-template <int H>
-__device__ __forceinline__ void nout_fetch(const FusedArgs& a, int t, int total, int k0,
-                                           u32x4 (&r)[Nout<H>::LD]) {
-  const uint16_t* W;
-  int ldw, row0;
-  nout_tile<H>(a, t < total ? t : total - 1, k0, &W, &ldw, &row0);
-  const int per_row = ldw / 8;  // 16-byte chunks per row
+// Registers -> LDS in the tile's row layout (layer 0: rows of K0, else of H).
+template <class F, int TL>
+__device__ __forceinline__ void fz_stage(uint16_t* T, const u32x4 (&r)[F::CH]) {
+  if constexpr (TL < F::NT) {
+    constexpr int per_row = (TL < F::T0 ? F::K0 : F::HID) / 8;  // 16-byte chunks per row
+    constexpr int stride = TL < F::T0 ? F::S0 : F::SH;
 #pragma unroll
-  for (int i = 0; i < Nout<H>::LD; ++i) {  // every register written (chunks past the tile reload chunk 0)
-    const int e = threadIdx.x + 256 * i < 32 * per_row ? threadIdx.x + 256 * i : 0;
-    r[i] = load_global16(W + (row0 + e / per_row) * ldw + 8 * (e % per_row));
+    for (int c = 0; c < F::CH; ++c) {
+      const int e = threadIdx.x + 256 * c;
+      *reinterpret_cast<u32x4*>(T + (e / per_row) * stride + 8 * (e % per_row)) = r[c];
+    }
   }
 }
 
-template <int H>
-__device__ __forceinline__ void nout_stage(int t, int k0, uint16_t (*T)[Nout<H>::TS],
-                                           const u32x4 (&r)[Nout<H>::LD]) {
-  const int per_row = t < Nout<H>::NB ? k0 / 8 : H / 8;
+// The 16 biases of block g (a later layer) this lane's accumulators start from.
+template <class F, int G>
+__device__ __forceinline__ void fz_bias(const float* b, f32x16& acc) {
+  constexpr int l = F::layer_of(G), nb = F::nb_of(G);
+  const float* p = b + (l - 1) * F::HID + 32 * nb + 4 * ((threadIdx.x & 63) >> 5);
+  f32x4 v[4];
 #pragma unroll
-  for (int i = 0; i < Nout<H>::LD; ++i) {
-    const int e = threadIdx.x + 256 * i;
-    if (e < 32 * per_row) *reinterpret_cast<u32x4*>(&T[e / per_row][8 * (e % per_row)]) = r[i];
-  }
-}
-
-// acc = W-tile (32 rows x 16 KS columns) x act[0..KS) for this wave's 32 samples.
-template <int H, int KS>
-__device__ __forceinline__ void nout_mfma(const uint16_t (*T)[Nout<H>::TS],
-                                          const bf16x8 (&act)[Nout<H>::KS], f32x16& acc) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-#pragma unroll
-  for (int ks = 0; ks < KS; ++ks) {
-    const bf16x8 w = *reinterpret_cast<const bf16x8*>(&T[lane & 31][16 * ks + 8 * (lane >> 5)]);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, act[ks], acc, 0, 0, 0);
-  }
-}
-
-// The 16 biases of output block nb this lane's accumulators need (features
-// 8q + 4h + u of the block).
-__device__ __forceinline__ void nout_bias(const float* bias, int nb, f32x4 (&bv)[4]) {
-  const int h = (threadIdx.x & 63) >> 5;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) bv[q] = load_global_f32x4(bias + 32 * nb + 8 * q + 4 * h);
-}
-
-// SiLU(acc + bias) of an output block -> the next layer's fragments 2 nb, 2 nb + 1.
-__device__ __forceinline__ void nout_epilogue(const f32x4 (&bv)[4], const f32x16& acc,
-                                              bf16x8& lo_frag, bf16x8& hi_frag) {
-  float v[16];
+  for (int q = 0; q < 4; ++q) v[q] = load_global_f32x4(p + 8 * q);
 #pragma unroll
   for (int q = 0; q < 4; ++q)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[4 * q + u] = silu(acc[4 * q + u] + bv[q][u]);
+    for (int u = 0; u < 4; ++u) acc[4 * q + u] = v[q][u];
+}
+
+// a = z' / (1 + 2^-z') of a block (z' = SCALE * acc) -> the next layer's fragments.
+template <bool SCALE>
+__device__ __forceinline__ void fz_epilogue(const f32x16& acc, bf16x8& lo_frag, bf16x8& hi_frag) {
+  float v[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const float z = SCALE ? acc[i] * kLog2e : acc[i];
+    v[i] = z * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-z));
+  }
   uint32_t P0 = pack_bf16(v[0], v[1]), P1 = pack_bf16(v[2], v[3]);
   uint32_t Q0 = pack_bf16(v[4], v[5]), Q1 = pack_bf16(v[6], v[7]);
   uint32_t R0 = pack_bf16(v[8], v[9]), R1 = pack_bf16(v[10], v[11]);
@@ -460,55 +453,21 @@ __device__ __forceinline__ void nout_epilogue(const f32x4 (&bv)[4], const f32x16
   hi_frag = __builtin_bit_cast(bf16x8, u32x4{x2[0], x3[0], x2[1], x3[1]});
 }
 
-// One layer of NB output blocks (KS k16-steps of input); t advances by NB (a
-// layer starts at an even t, so the parity of t is that of the block).  Tile
-// t + 1 is in registers (r1 for odd t + 1, r0 for even) when block t starts: its
-// global loads were issued a whole block earlier; block t issues tile t + 2's.
-template <int H, int KS>
-__device__ __forceinline__ void nout_layer(const FusedArgs& a, int k0, const float* bias,
-                                           uint16_t (*tile)[32][Nout<H>::TS], int& t, int total,
-                                           const bf16x8 (&act)[Nout<H>::KS],
-                                           bf16x8 (&nxt)[Nout<H>::KS], u32x4 (&r0)[Nout<H>::LD],
-                                           u32x4 (&r1)[Nout<H>::LD]) {
-  constexpr int NB = Nout<H>::NB;
-  f32x16 prev;
-  static_for<0, NB>([&](auto Bc) {
-    constexpr int nb = decltype(Bc)::value;
-    auto block = [&](u32x4 (&next)[Nout<H>::LD], u32x4 (&after)[Nout<H>::LD]) {  // t + 1, t + 2
-      __syncthreads();  // tile t staged; the other buffer is free
-      // The previous block's biases are loaded before tile t + 2's weights, so
-      // waiting for them does not wait for the weight prefetch.
-      f32x4 bv[4];
-      if constexpr (nb > 0) nout_bias(bias, nb - 1, bv);
-      nout_fetch<H>(a, t + 2, total, k0, after);
-      f32x16 cur;
-      nout_mfma<H, KS>(tile[t & 1], act, cur);
-      if constexpr (nb > 0) nout_epilogue(bv, prev, nxt[2 * nb - 2], nxt[2 * nb - 1]);
-      if (t + 1 < total) nout_stage<H>(t + 1, k0, tile[(t + 1) & 1], next);
-      prev = cur;
-      ++t;
-    };
-    if constexpr (nb & 1)
-      block(r0, r1);
-    else
-      block(r1, r0);
-  });
-  f32x4 bv[4];
-  nout_bias(bias, NB - 1, bv);
-  nout_epilogue(bv, prev, nxt[2 * NB - 2], nxt[2 * NB - 1]);
-}
-
-// K0S = k16 steps of the padded input width (round64(w0) / 16).
-template <int H, int K0S>
-__global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward_nout(const FusedArgs a) {
-  using NO = Nout<H>;
-  __shared__ __align__(16) uint16_t tile[2][32][NO::TS];
+template <int H, int K0S, int L>
+__global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const FusedArgs a) {
+  using F = Fz<H, K0S, L>;
+  __shared__ __align__(16) uint16_t lds[2][F::LDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
   const int64_t m = static_cast<int64_t>(blockIdx.x) * 128 + wave * 32 + ln;  // this lane's sample
-  constexpr int k0 = 16 * K0S;
-  bf16x8 act[NO::KS], nxt[NO::KS];
+
+  u32x4 rg[F::PD][F::CH];
+  fz_fetch<F, 0>(a.W, rg[0]);
+  // inputs, then the two constant-1 columns of layer 0's bias (branch free: every
+  // load is in bounds, rows past the end read the last row and are zeroed)
+  bf16x8 xin[K0S];
+  const float* xrow = a.x + (m < a.rows ? m : a.rows - 1) * a.w0;
 #pragma unroll
-  for (int ks = 0; ks < NO::KS; ++ks) {
+  for (int ks = 0; ks < K0S; ++ks) {
     uint32_t d[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -516,43 +475,107 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward_nout(
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int k = 16 * ks + 8 * h + 2 * j + u;
-        f[u] = (ks < K0S && k < a.w0 && m < a.rows) ? a.x[m * a.w0 + k] : 0.0f;
+        const float xv = xrow[k < a.w0 ? k : a.w0 - 1];
+        const float fv = k < a.w0 ? xv : (k < a.w0 + 2 ? 1.0f : 0.0f);
+        f[u] = m < a.rows ? fv : 0.0f;
       }
       d[j] = pack_bf16(f[0], f[1]);
     }
-    act[ks] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
+    xin[ks] = __builtin_bit_cast(bf16x8, u32x4{d[0], d[1], d[2], d[3]});
   }
-  const int total = NO::NB * (a.layers - 1) + 1;
-  int t = 0;
-  u32x4 r0[NO::LD], r1[NO::LD];
-  nout_fetch<H>(a, 0, total, k0, r0);
-  nout_stage<H>(0, k0, tile[0], r0);
-  nout_fetch<H>(a, 1, total, k0, r1);  // tile 1 in flight
-  nout_layer<H, K0S>(a, k0, layer_b(a, 0), tile, t, total, act, nxt, r0, r1);
-  for (int l = 1; l + 1 < a.layers; ++l) {
+  fz_stage<F, 0>(lds[0], rg[0]);
+  if constexpr (F::PD == 2) fz_fetch<F, 1>(a.W, rg[1]);
+
+  bf16x8 act0[F::KS], act1[F::KS];  // layer l's output: act0 for even l, act1 for odd
+  f32x16 prev, bnext, bcur;
+  static_for<0, F::NT>([&](auto Tc) {
+    constexpr int t = decltype(Tc)::value;
+    __syncthreads();  // tile t staged in lds[t & 1]; the other buffer is free
+    // biases before the weight prefetch, so waiting for them never waits for weights
+    if constexpr (F::PD == 2) {
+      if constexpr (t + 1 < F::NT && t + 1 >= F::T0) fz_bias<F, F::first_of(t + 1)>(a.b, bnext);
+      fz_fetch<F, t + 2>(a.W, rg[t & 1]);
+    } else {
+      if constexpr (t >= F::T0) fz_bias<F, F::first_of(t)>(a.b, bcur);
+      fz_fetch<F, t + 1>(a.W, rg[0]);
+    }
+    const uint16_t* T = lds[t & 1];
+    static_for<0, F::blocks_of(t)>([&](auto Jc) {
+      constexpr int g = F::first_of(t) + decltype(Jc)::value;
+      constexpr int layer = F::layer_of(g);
+      constexpr int ksteps = layer == 0 ? K0S : F::KS;
+      constexpr int stride = layer == 0 ? F::S0 : F::SH;
+      // The previous block's epilogue.  Inside a layer this block's MFMA chain does
+      // not depend on it, so the scheduler interleaves the two; the first block of
+      // a layer reads its last two fragments, so there it comes first (program
+      // order), and only the last two steps of the chain wait for it.
+      auto prev_epilogue = [&]() {
+        if constexpr (g > 0) {
+          constexpr int pl = F::layer_of(g - 1), pnb = F::nb_of(g - 1);
+          auto& out = pick<pl & 1>(act0, act1);
+          fz_epilogue<pl == 0>(prev, out[2 * pnb], out[2 * pnb + 1]);
+        }
+      };
+      constexpr bool first_of_layer = g > 0 && F::layer_of(g - 1) != layer;
+      if constexpr (first_of_layer) prev_epilogue();
+      f32x16 acc;
+      if constexpr (layer == 0) {
 #pragma unroll
-    for (int ks = 0; ks < NO::KS; ++ks) act[ks] = nxt[ks];
-    nout_layer<H, NO::KS>(a, k0, layer_b(a, l), tile, t, total, act, nxt, r0, r1);
-  }
-  __syncthreads();  // output layer tile staged
-  f32x16 out;
-  nout_mfma<H, NO::KS>(tile[t & 1], nxt, out);
-  const float* bias = layer_b(a, a.layers - 1);
+        for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
+      } else {
+        acc = bcur;
+      }
+      const uint16_t* arow = T + (32 * F::pos_of(g) + ln) * stride + 8 * h;
+#pragma unroll
+      for (int ks = 0; ks < ksteps; ++ks) {
+        const bf16x8 w = *reinterpret_cast<const bf16x8*>(arow + 16 * ks);
+        if constexpr (layer == 0)
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xin[ks], acc, 0, 0, 0);
+        else
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+              w, pick<(layer - 1) & 1>(act0, act1)[ks], acc, 0, 0, 0);
+      }
+      if constexpr (!first_of_layer) prev_epilogue();
+      prev = acc;
+    });
+    if constexpr (t + 1 < F::NT) fz_stage<F, t + 1>(lds[(t + 1) & 1], rg[(t + 1) % F::PD]);
+    if constexpr (F::PD == 2) bcur = bnext;
+  });
+  // prev holds the output block: log2(e) (W_out a + b_out) with a = log2(e) SiLU
   if (m < a.rows) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n = 8 * (r >> 2) + 4 * h + (r & 3);
-      if (n < a.w_out) a.y[m * a.w_out + n] = out[r] + bias[n];
+      if (n < a.w_out) a.y[m * a.w_out + n] = prev[r] * (1.0f / kLog2e);
     }
   }
 }
 
-// Padded bf16 copy [round32(out)][round64(in)] of an fp32 [out][in] matrix.
-__global__ void pad_bf16(const float* w, int out, int in, int out_p, int in_p, uint16_t* dst) {
+// One layer's part of the weight stream: [out_p][in_p] bf16 of an fp32 [out][in]
+// matrix, zero padded; with `bias_cols`, columns in and in + 1 hold the bf16
+// high and low parts of b (layer 0's bias against constant-1 inputs).
+__global__ void fz_pack_weights(const float* w, const float* b, int out, int in, int out_p,
+                                int in_p, int bias_cols, uint16_t* dst) {
   const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (i >= int64_t(out_p) * in_p) return;
   const int n = static_cast<int>(i / in_p), k = static_cast<int>(i % in_p);
-  dst[i] = (n < out && k < in) ? f32_to_bf16_bits(w[int64_t(n) * in + k]) : uint16_t(0);
+  uint16_t v = 0;
+  if (n < out) {
+    if (k < in) {
+      v = f32_to_bf16_bits(w[int64_t(n) * in + k]);
+    } else if (bias_cols && k < in + 2) {
+      const uint16_t hi = f32_to_bf16_bits(b[n]);
+      const float hf = __builtin_bit_cast(float, static_cast<uint32_t>(hi) << 16);
+      v = k == in ? hi : f32_to_bf16_bits(b[n] - hf);
+    }
+  }
+  dst[i] = v;
+}
+
+// s * b of one layer, zero padded to n_p entries.
+__global__ void fz_scale_bias(const float* b, int n, int n_p, float s, float* dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_p) dst[i] = i < n ? s * b[i] : 0.0f;
 }
 
 }  // namespace
@@ -570,9 +593,12 @@ struct ce_mlp {
   std::vector<uint16_t*> Wb;  // bf16 copies [out][in]
   int fused_h = 0;            // hidden width of the fused inference kernel, 0 = none
   bool use_fused = true;      // inference (ce_mlp_forward, closure) through it
-  bool wp_stale = true;       // Wp older than the master parameters
-  uint16_t* Wp_all = nullptr;  // padded bf16 copies for the fused kernel, one allocation
-  std::vector<uint16_t*> Wp;   // layer l's copy inside Wp_all
+  bool wp_stale = true;       // the fused kernel's weight stream older than the master
+  uint16_t* Wp_all = nullptr;  // the fused kernel's weight stream (all layers back to back)
+  std::vector<uint16_t*> Wp;   // layer l's part of it
+  float* Bs = nullptr;         // the fused kernel's scaled biases of layers 1 .. L - 1
+  float *fx = nullptr, *fy = nullptr;  // ce_mlp_forward's device input / output through it
+  int64_t fcap = 0;
   int64_t adam_t = 0;
   // batch workspace
   int64_t cap = 0;
@@ -621,7 +647,7 @@ void free_mlp(ce_mlp* m) {
   f(m->P), f(m->G), f(m->M1), f(m->M2), f(m->out), f(m->dY), f(m->dZ32), f(m->dZ16);
   f(m->loss), f(m->xin), f(m->tgt), f(m->ds_x), f(m->ds_t), f(m->perm);
   for (auto* p : m->Wb) f(p);
-  f(m->Wp_all);
+  f(m->Wp_all), f(m->Bs), f(m->fx), f(m->fy);
   for (auto* p : m->act) f(p);
   for (auto* p : m->Z) f(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -839,27 +865,30 @@ int ce_mlp_create(const int32_t* widths, int32_t n_widths, uint64_t seed, ce_mlp
   m->Wb.assign(m->L, nullptr);
   for (int l = 0; l < m->L && rc == CE_OK; ++l)
     rc = dalloc(&m->Wb[l], static_cast<int64_t>(widths[l]) * widths[l + 1]);
-  // the fused inference kernel covers [w0, H, ..., H, w_out <= 32] with H = 256 and
-  // w0 <= 64, or H = 512 and w0 <= 128
+  // the fused inference kernel covers [w0, H, ..., H, w_out <= 32] with H = 256,
+  // w0 + 2 <= 64 and up to 6 layers, or H = 512, w0 + 2 <= 128 and up to 5 layers
   {
     const int h = m->L >= 2 ? widths[1] : 0;
-    bool ok = ((h == 256 && widths[0] <= 64) || (h == 512 && widths[0] <= 128)) &&
-              widths[m->L] <= 32 && m->L <= kFusedMaxLayers;
+    bool ok = ((h == 256 && widths[0] + 2 <= 64 && m->L <= kFusedMaxLayers) ||
+               (h == 512 && widths[0] + 2 <= 128 && m->L <= kFusedMaxLayers512)) &&
+              widths[m->L] <= 32;
     for (int l = 1; l < m->L && ok; ++l) ok = widths[l] == h;
     m->fused_h = ok ? h : 0;
   }
   if (m->fused_h) {
-    // Layers back to back: [H][round64(w0)], then (L - 2) x [H][H], then [32][H], so
-    // the kernel finds layer l >= 1 at a uniform stride from the first.
+    // The weight stream: [H][round64(w0 + 2)] (two bias columns), then (L - 2) x
+    // [H][H], then [32][H], back to back; the scaled biases of layers 1 .. L - 1.
     m->Wp.assign(m->L, nullptr);
     int64_t total = 0;
     std::vector<int64_t> off(m->L);
     for (int l = 0; l < m->L; ++l) {
       off[l] = total;
-      total += static_cast<int64_t>(round_up(widths[l + 1], 32)) * round_up(widths[l], 64);
+      total += static_cast<int64_t>(round_up(widths[l + 1], 32)) *
+               (l == 0 ? round_up(widths[0] + 2, 64) : widths[l]);
     }
     if (rc == CE_OK) rc = dalloc(&m->Wp_all, total);
     for (int l = 0; l < m->L && rc == CE_OK; ++l) m->Wp[l] = m->Wp_all + off[l];
+    if (rc == CE_OK) rc = dalloc(&m->Bs, static_cast<int64_t>(m->L - 1) * m->fused_h);
   }
   if (rc == CE_OK) {
     std::vector<float> p;
@@ -912,18 +941,33 @@ int ce_mlp_set_params(ce_mlp* m, const float* host) {
 int ce_mlp_forward(ce_mlp* m, const float* x_host, size_t batch, float* y_host) {
   if (!m || (!x_host && batch) || (!y_host && batch)) return fail(CE_ERR_INVALID_ARGUMENT, "NULL argument");
   MLP_HIP(hipSetDevice(m->device));
-  const int64_t chunk = 65536;
+  // rows per device pass: the fused kernel needs only input and output buffers
+  const bool fused = m->fused_h && m->use_fused;
+  const int64_t chunk = fused ? (1 << 20) : 65536;
   for (size_t b0 = 0; b0 < batch; b0 += chunk) {
     const int64_t B = static_cast<int64_t>(std::min<size_t>(chunk, batch - b0));
-    int rc = ensure_batch(m, B);
+    int rc = CE_OK;
+    if (fused && B > m->fcap) {
+      if (m->fx) (void)hipFree(m->fx);
+      if (m->fy) (void)hipFree(m->fy);
+      m->fx = m->fy = nullptr;
+      m->fcap = 0;
+      rc = dalloc(&m->fx, B * m->widths[0]);
+      if (rc == CE_OK) rc = dalloc(&m->fy, B * m->widths[m->L]);
+      if (rc == CE_OK) m->fcap = B;
+    } else if (!fused) {
+      rc = ensure_batch(m, B);
+    }
     if (rc != CE_OK) return rc;
-    MLP_HIP(hipMemcpyAsync(m->xin, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
+    float* xd = fused ? m->fx : m->xin;
+    float* yd = fused ? m->fy : m->out;
+    MLP_HIP(hipMemcpyAsync(xd, x_host + b0 * m->widths[0], B * m->widths[0] * sizeof(float),
                            hipMemcpyHostToDevice, m->stream));
     ce::clear_stale_error();
-    rc = (m->fused_h && m->use_fused) ? ce::mlp_forward_device(m, m->xin, B, m->out, m->stream)
-                                      : forward(m, m->xin, B, false, m->out, m->stream);
+    rc = fused ? ce::mlp_forward_device(m, xd, B, yd, m->stream)
+               : forward(m, xd, B, false, yd, m->stream);
     if (rc != CE_OK) return rc;
-    MLP_HIP(hipMemcpyAsync(y_host + b0 * m->widths[m->L], m->out,
+    MLP_HIP(hipMemcpyAsync(y_host + b0 * m->widths[m->L], yd,
                            B * m->widths[m->L] * sizeof(float), hipMemcpyDeviceToHost, m->stream));
   }
   MLP_HIP(hipStreamSynchronize(m->stream));
@@ -1087,12 +1131,17 @@ int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
   ce::clear_stale_error();
   if (m->fused_h && m->use_fused && B > 0) {
+    const int H = m->fused_h, L = m->L, k0 = round_up(m->widths[0] + 2, 64);
     if (m->wp_stale) {
-      for (int l = 0; l < m->L; ++l) {
+      for (int l = 0; l < L; ++l) {
         const int in = m->widths[l], out = m->widths[l + 1];
-        const int op = round_up(out, 32), ip = round_up(in, 64);
-        hipLaunchKernelGGL(pad_bf16, dim3(blocks_for(int64_t(op) * ip)), dim3(256), 0, s,
-                           m->P + m->w_off[l], out, in, op, ip, m->Wp[l]);
+        const int op = round_up(out, 32), ip = l == 0 ? k0 : in;
+        hipLaunchKernelGGL(fz_pack_weights, dim3(blocks_for(int64_t(op) * ip)), dim3(256), 0, s,
+                           m->P + m->w_off[l], m->P + m->b_off[l], out, in, op, ip, l == 0 ? 1 : 0,
+                           m->Wp[l]);
+        if (l > 0)
+          hipLaunchKernelGGL(fz_scale_bias, dim3(blocks_for(H)), dim3(256), 0, s,
+                             m->P + m->b_off[l], out, H, kLog2e, m->Bs + int64_t(l - 1) * H);
       }
       m->wp_stale = false;
     }
@@ -1100,19 +1149,22 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     a.x = x;
     a.rows = B;
     a.w0 = m->widths[0];
-    a.layers = m->L;
-    a.w_out = m->widths[m->L];
+    a.w_out = m->widths[L];
     a.W = m->Wp_all;
-    for (int l = 0; l < m->L; ++l) a.b[l] = m->P + m->b_off[l];
+    a.b = m->Bs;
     a.y = y;
-    // 32 samples per wave, 4 waves per workgroup, output-block-major
-    const unsigned grid = static_cast<unsigned>((B + 127) / 128);
-    if (m->fused_h == 256)
-      hipLaunchKernelGGL((mlp_fused_forward_nout<256, 4>), dim3(grid), dim3(256), 0, s, a);
-    else if (round_up(a.w0, 64) == 64)
-      hipLaunchKernelGGL((mlp_fused_forward_nout<512, 4>), dim3(grid), dim3(256), 0, s, a);
-    else
-      hipLaunchKernelGGL((mlp_fused_forward_nout<512, 8>), dim3(grid), dim3(256), 0, s, a);
+    // 32 samples per wave, 4 waves per workgroup
+    const dim3 grid(static_cast<unsigned>((B + 127) / 128)), blk(256);
+    using Kern = void (*)(const FusedArgs);
+    static const Kern k256[] = {mlp_fused_forward<256, 4, 2>, mlp_fused_forward<256, 4, 3>,
+                                mlp_fused_forward<256, 4, 4>, mlp_fused_forward<256, 4, 5>,
+                                mlp_fused_forward<256, 4, 6>};
+    static const Kern k512a[] = {mlp_fused_forward<512, 4, 2>, mlp_fused_forward<512, 4, 3>,
+                                 mlp_fused_forward<512, 4, 4>, mlp_fused_forward<512, 4, 5>};
+    static const Kern k512b[] = {mlp_fused_forward<512, 8, 2>, mlp_fused_forward<512, 8, 3>,
+                                 mlp_fused_forward<512, 8, 4>, mlp_fused_forward<512, 8, 5>};
+    const Kern kern = H == 256 ? k256[L - 2] : (k0 == 64 ? k512a[L - 2] : k512b[L - 2]);
+    hipLaunchKernelGGL(kern, grid, blk, 0, s, a);
     MLP_HIP(hipGetLastError());
     return CE_OK;
   }

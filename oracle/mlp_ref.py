@@ -58,6 +58,34 @@ def forward(params, widths, x, exact=True):
     return a, (acts, zs, layers)
 
 
+LOG2E = 1.4426950408889634
+
+
+def forward_log2_scaled(params, widths, x, exact=True):
+    """The fused inference kernel's arithmetic (csrc/mlp.hip, mlp_fused_forward):
+    hidden layers carry a = log2(e) SiLU(z), rounded to bf16 (exact=False), their
+    biases enter as log2(e) b (float32), and SiLU is z' / (1 + 2^-z') on the
+    log2-scaled sums z'; layer 0's bias is the bf16 high + low split of b against
+    two constant-1 inputs; the output is (W_out a + log2(e) b_out) / log2(e).
+    With exact=True every rounding is off and it equals forward(exact=True)."""
+    rnd = (lambda a: a) if exact else (lambda a: bf16(a.astype(np.float32)))
+    f32 = (lambda a: a) if exact else (lambda a: np.asarray(a, np.float32).astype(np.float64))
+    layers = unpack(params, widths)
+    a = rnd(np.asarray(x, dtype=np.float64))
+    for i, (w, b) in enumerate(layers):
+        if i == 0:
+            b_hi = rnd(b)
+            b_lo = rnd(b - b_hi)
+            zs = LOG2E * (a @ rnd(w).T + (b_hi + b_lo))
+        else:
+            zs = a @ rnd(w).T + f32(LOG2E * b)
+        if i + 1 < len(layers):
+            a = rnd(zs / (1.0 + np.exp2(-zs)))
+        else:
+            a = zs / LOG2E
+    return a
+
+
 def loss_and_grad(params, widths, x, t, exact=True):
     rnd = (lambda a: a) if exact else (lambda a: bf16(a.astype(np.float32)))
     y, (acts, zs, layers) = forward(params, widths, x, exact)

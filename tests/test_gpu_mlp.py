@@ -58,6 +58,7 @@ def test_forward_and_gradients_vs_oracle(gpu, widths, rows):
     np.testing.assert_array_equal(net.params(), p)
     x = rng.normal(size=(rows, widths[0])).astype(np.float32)
     t = rng.normal(size=(rows, widths[-1])).astype(np.float32)
+    net.set_fused(False)  # the training path's layer-by-layer GEMMs (the fused kernel: below)
     y = net.forward(x)
     yb, _ = mlp_ref.forward(p, widths, x, exact=False)
     # Same bf16 roundings as the oracle; differences are FP32 accumulation order and
@@ -148,8 +149,19 @@ def test_mlp_argument_errors(gpu):
                                          ([80, 512, 512, 512, 512, 30], 300),
                                          ([50, 512, 512, 512, 9], 200),
                                          ([13, 256, 256, 6], 257)])
-def test_fused_forward_same_bits_as_layer_by_layer(gpu, widths, rows):
-    """The fused inference kernel accumulates in the GEMM's K order: same bits."""
+def test_fused_forward_vs_oracle(gpu, widths, rows):
+    """The fused inference kernel against the oracle's statement of its arithmetic
+    (log2-scaled bf16 activations, layer 0's bias as a bf16 pair): equal up to the
+    FP32 summation order and the 1-ulp exp2 / reciprocal.  Those differences are
+    ~1e-8 of the output scale, except where they move a hidden activation across a
+    bf16 rounding boundary (one step, 2^-8 of it; about one activation in 10^4),
+    which shifts the outputs downstream of it by up to ~1e-3 of the scale (the
+    oracle with float32 instead of exact sums shows the same: for the 5-layer
+    512-wide net the 99th percentile of that shift is 3.2e-4 of the scale, 1.5e-4
+    for the layer-by-layer form).  The bar: median within 1e-6 (a systematic error
+    would move it), 99th percentile within 1e-3, largest within 1e-2 of the scale.
+    Against the layer-by-layer path (SiLU rounded to bf16 unscaled): bf16 rounding
+    of the activations."""
     ce = gpu
     rng = np.random.default_rng(rows)
     net = ce.MLP(widths, seed=9)
@@ -160,13 +172,29 @@ def test_fused_forward_same_bits_as_layer_by_layer(gpu, widths, rows):
     y_fused = net.forward(x)
     assert not net.set_fused(False)
     y_layers = net.forward(x)
-    np.testing.assert_array_equal(y_fused, y_layers)
+    scale = np.abs(y_layers).max()
+    yf_ref = mlp_ref.forward_log2_scaled(p, widths, x, exact=False)
+    d = np.abs(y_fused - yf_ref)
     yb, _ = mlp_ref.forward(p, widths, x, exact=False)
-    assert np.abs(y_fused - yb).max() < 1e-2 * np.abs(yb).max()
+    d_layers = np.abs(y_layers - yb)
+    assert d_layers.max() < 1e-2 * scale
+    assert d.max() < 1e-2 * scale
+    assert np.quantile(d, 0.99) < 1e-3 * scale
+    assert np.median(d) < 1e-6 * scale
+    assert np.abs(y_fused - y_layers).max() < 2e-2 * scale
+    # a row's result depends on its own inputs only: a call over the first rows
+    # (a partial last workgroup) gives the same bits for them
+    if rows > 130:
+        assert net.set_fused(True)
+        np.testing.assert_array_equal(net.forward(x[:rows - 129]), y_fused[:rows - 129])
 
 
 def test_fused_not_used_for_other_shapes(gpu):
     ce = gpu
     assert not ce.MLP([7, 64, 50, 3]).set_fused(True)
-    assert not ce.MLP([100, 256, 256, 8]).set_fused(True)  # input wider than 64 at H = 256
+    assert not ce.MLP([100, 256, 256, 8]).set_fused(True)  # input wider than 62 at H = 256
+    assert not ce.MLP([63, 256, 256, 8]).set_fused(True)   # no room for the bias columns
+    assert ce.MLP([62, 256, 256, 8]).set_fused(True)
+    assert not ce.MLP([40] + [256] * 6 + [8]).set_fused(True)  # 7 weight layers
+    assert not ce.MLP([40] + [512] * 5 + [8]).set_fused(True)  # 6 at H = 512
     assert not ce.MLP([130, 512, 512, 8]).set_fused(True)  # input wider than 128 at H = 512

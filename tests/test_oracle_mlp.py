@@ -59,3 +59,24 @@ def test_bf16_mode_close_to_exact():
     _, ge = mlp_ref.loss_and_grad(p, widths, x, t, exact=True)
     _, gb = mlp_ref.loss_and_grad(p, widths, x, t, exact=False)
     assert np.linalg.norm(ge - gb) < 0.05 * np.linalg.norm(ge)
+
+
+def test_log2_scaled_form_equals_the_network():
+    """The fused kernel's reformulation (log2-scaled activations and biases, layer
+    0's bias as a high + low pair) is the same function: exact mode equals the
+    plain forward; with bf16 rounding both stay at bf16-level distance."""
+    rng = np.random.default_rng(11)
+    for widths in ([40, 256, 256, 256, 15], [13, 64, 64, 6], [80, 512, 512, 30]):
+        p = mlp_ref.glorot_params(widths, rng)
+        p[-widths[-1]:] = rng.normal(size=widths[-1])  # output biases of order 1
+        x = rng.normal(size=(64, widths[0]))
+        y, _ = mlp_ref.forward(p, widths, x, exact=True)
+        y2 = mlp_ref.forward_log2_scaled(p, widths, x, exact=True)
+        np.testing.assert_allclose(y2, y, rtol=0, atol=1e-12 * np.abs(y).max())
+        yb, _ = mlp_ref.forward(p, widths, x, exact=False)
+        y2b = mlp_ref.forward_log2_scaled(p, widths, x, exact=False)
+        scale = np.abs(y).max()
+        assert np.abs(y2b - y).max() < 3e-2 * scale
+        assert np.abs(y2b - yb).max() < 3e-2 * scale
+        # a dropped log2(e) or bias would be far off
+        assert np.abs(mlp_ref.forward_log2_scaled(p * 1.01, widths, x) - y).max() > 1e-3 * scale
