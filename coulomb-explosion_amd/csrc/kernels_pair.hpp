@@ -26,12 +26,24 @@
 namespace ce {
 
 // The partner lane's value (lanes 2k <-> 2k+1, DPP quad_perm [1,0,3,2]).  Both
-// lanes of a pair are always active together.
+// lanes of a pair are always active together.  bound_ctrl is set: the swap never
+// reads outside its row, so it changes no value, and with it the compiler folds
+// the move into the instruction that uses it (v_sub_f32_dpp, v_add_f32_dpp): a
+// partner operand costs nothing and holds no register.
 __device__ __forceinline__ float xchg(float v) {
   return __builtin_bit_cast(
-      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, false));
+      float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0xB1, 0xF, 0xF, true));
 }
-__device__ __forceinline__ int xchg(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ int xchg(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true); }
+// xchg(v) as the operand of one subtraction a - partner(v): `a` is passed as the
+// move's (ignored) old value, so the uses of one partner value are distinct
+// calls and each folds into its own v_sub_f32_dpp instead of sharing one held
+// register.
+__device__ __forceinline__ float xchg_for(float v, float a) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, a),
+                                                               __builtin_bit_cast(int, v), 0xB1,
+                                                               0xF, 0xF, true));
+}
 __device__ __forceinline__ double xchg(double v) {
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
   const uint32_t lo = static_cast<uint32_t>(xchg(static_cast<int>(static_cast<uint32_t>(b))));
@@ -44,76 +56,84 @@ __device__ __forceinline__ V pair_sum(V a) { return a + xchg(a); }
 
 template <int N, int CM>
 constexpr bool use_pair_lanes() {
-  return N == 8 && (CM == kPerSim || CM == kUniform);
+  return (N == 8 || N == 10) && (CM == kPerSim || CM == kUniform);
 }
 
-// Pair constants of a lane: its own atoms' pairs and its cross pairs.
+// Charges of a lane's pair terms.  Per-sim charges: the force sums carry
+// G_i = sum_j q_j r_ij / |r_ij|^3 and the per-atom factor k q_i / m_i (the lane's
+// im, set when a sim starts) turns them into accelerations, as the uniform form's
+// c / m does, so a pair term needs q_j / |r|^3 and q_i / |r|^3 only and no pair
+// constant is formed or held.  Uniform pair constant c: the factors are 1 and c
+// is in im.
 template <int NA, int CM>
 struct PairLaneK {
-  float kqo[CM == kPerSim ? NA : 1];  // T(k q_i), own atoms
-  float qo[CM == kPerSim ? NA : 1];   // T(q_i), own atoms
-  float qp[CM == kPerSim ? NA : 1];   // T(q_j), partner atoms
-  float pc0 = 1.0f;                   // kUniform: the common pair constant
-  double pcd0 = 1.0;                  // kUniform: the same in FP64
+  float qo[CM == kPerSim ? NA : 1];  // q_i of the own atoms (the partner's: from its lane)
+  float pc0 = 1.0f;                  // kUniform: the common pair constant
+  double pcd0 = 1.0;                 // kUniform: the same in FP64
+  double kd = 1.0;                   // kPerSim: the Coulomb constant
   // Charges of the sim starting at qs (its N = 2·NA per-atom charges).
-  __device__ __forceinline__ void load_charges(const float* qs, int h, double coulomb_k) {
+  __device__ __forceinline__ void load_charges(const float* qs, int h) {
     if constexpr (CM == kPerSim) {
 #pragma unroll
-      for (int i = 0; i < NA; ++i) {
-        const float own = qs[h * NA + i];
-        qo[i] = own;
-        kqo[i] = static_cast<float>(coulomb_k * static_cast<double>(own));
-        qp[i] = qs[(1 - h) * NA + i];
-      }
+      for (int i = 0; i < NA; ++i) qo[i] = qs[h * NA + i];
     }
   }
-  template <int I, int J>
-  __device__ __forceinline__ float own() const {
-    if constexpr (CM == kPerSim) return kqo[I] * qo[J];
+  template <int J>
+  __device__ __forceinline__ float q_own() const {
+    if constexpr (CM == kPerSim) return qo[J];
     else return 1.0f;
   }
-  template <int I, int J>
-  __device__ __forceinline__ float cross() const {
-    if constexpr (CM == kPerSim) return kqo[I] * qp[J];
+  template <int J>
+  __device__ __forceinline__ float q_partner() const {
+    if constexpr (CM == kPerSim) return xchg(qo[J]);
     else return 1.0f;
   }
-  __device__ __forceinline__ float pe(float u) const { return CM == kUniform ? u * pc0 : u; }
-  // FP64 pair constants (energy diagnostic): the exact product of the two
-  // factors for per-sim charges, the molecule's FP64 constant otherwise.
+  // sum q_i q_j / r -> the potential energy
+  __device__ __forceinline__ float pe(float u) const {
+    if constexpr (CM == kUniform) return u * pc0;
+    else if constexpr (CM == kPerSim) return u * static_cast<float>(kd);
+    else return u;
+  }
+  // FP64 pair constants (energy diagnostic): float(k q_i) q_j exactly for per-sim
+  // charges, the molecule's FP64 constant otherwise.
   template <int I, int J>
   __device__ __forceinline__ double own_d() const {
-    if constexpr (CM == kPerSim) return static_cast<double>(kqo[I]) * static_cast<double>(qo[J]);
+    if constexpr (CM == kPerSim)
+      return static_cast<double>(static_cast<float>(kd * static_cast<double>(qo[I]))) *
+             static_cast<double>(qo[J]);
     else return pcd0;
   }
   template <int I, int J>
   __device__ __forceinline__ double cross_d() const {
-    if constexpr (CM == kPerSim) return static_cast<double>(kqo[I]) * static_cast<double>(qp[J]);
+    if constexpr (CM == kPerSim)
+      return static_cast<double>(static_cast<float>(kd * static_cast<double>(qo[I]))) *
+             static_cast<double>(xchg(qo[J]));
     else return pcd0;
   }
 };
 
-// One pair term: r = xa - xb, s = c / |r|^3; returns 1/|r|.
-__device__ __forceinline__ float pair_term(const float* xa, const float* xb, float c, float (&r)[3],
-                                           float& s) {
-  r[0] = xa[0] - xb[0];
-  r[1] = xa[1] - xb[1];
-  r[2] = xa[2] - xb[2];
+// One pair term: r = xa - xb; returns 1/|r|^3 and sets inv = 1/|r|.  With
+// PARTNER, xb are the partner lane's coordinates (each fetched by the
+// subtraction itself).
+template <bool PARTNER = false>
+__device__ __forceinline__ float pair_term(const float* xa, const float* xb, float (&r)[3],
+                                           float& inv) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) r[d] = xa[d] - (PARTNER ? xchg_for(xb[d], xa[d]) : xb[d]);
   const float d2 = fmaT(r[0], r[0], fmaT(r[1], r[1], r[2] * r[2]));
-  const float inv = rsqrtT(d2);
-  s = c * ((inv * inv) * inv);
-  return inv;
+  inv = rsqrtT(d2);
+  return (inv * inv) * inv;
 }
 
-// Forces on the lane's own atoms (summed over all N atoms; no 1/m) and, with
-// WITH_PE, the sim's potential energy (both halves combined).
+// Force sums G of the lane's own atoms (over all N atoms; the per-atom factor
+// im applies later) and, with WITH_PE, the sim's potential energy (both halves).
 template <int NA, int CM, bool WITH_PE>
 __device__ __forceinline__ void pair_force(const float (&xo)[3 * NA], float (&a)[3 * NA], float& pe,
                                            const PairLaneK<NA, CM>& pk) {
   constexpr int NA3 = 3 * NA;
-  float xp[NA3], ap[NA3];
+  float ap[NA3];  // reactions on the partner's atoms
 #pragma unroll
   for (int q = 0; q < NA3; ++q) {
-    xp[q] = xchg(xo[q]);
     a[q] = 0.0f;
     ap[q] = 0.0f;
   }
@@ -123,61 +143,59 @@ __device__ __forceinline__ void pair_force(const float (&xo)[3 * NA], float (&a)
     constexpr int i = decltype(I)::value;
     static_for<i + 1, NA>([&](auto J) {
       constexpr int j = decltype(J)::value;
-      const float c = pk.template own<i, j>();
-      float r[3], s;
-      const float inv = pair_term(&xo[3 * i], &xo[3 * j], c, r, s);
-      if constexpr (WITH_PE) u = fmaT(c, inv, u);
+      const float qi = pk.template q_own<i>(), qj = pk.template q_own<j>();
+      float r[3], inv;
+      const float inv3 = pair_term(&xo[3 * i], &xo[3 * j], r, inv);
+      const float si = qj * inv3, sj = qi * inv3;
+      if constexpr (WITH_PE) u = fmaT(qi * qj, inv, u);
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
-        a[3 * j + d] = fmaT(-r[d], s, a[3 * j + d]);
+        a[3 * i + d] = fmaT(r[d], si, a[3 * i + d]);
+        a[3 * j + d] = fmaT(-r[d], sj, a[3 * j + d]);
       }
     });
   });
-  // cross pairs at offsets 1..(NA-1)/2: both sides, the reaction to the partner
+  // cross pairs with the partner's atom j: own atom i at offsets 1..(NA-1)/2 both
+  // sides (the reaction to the partner; the partner lane covers the opposite
+  // offsets), and for even NA at offset NA/2 for own atoms i < NA/2 (the partner
+  // lane's i < NA/2 are the other half of those pairs)
+  auto cross = [&](auto Ic, auto Jc) {
+    constexpr int i = decltype(Ic)::value, j = decltype(Jc)::value;
+    const float qi = pk.template q_own<i>(), qj = pk.template q_partner<j>();
+    float r[3], inv;
+    const float inv3 = pair_term<true>(&xo[3 * i], &xo[3 * j], r, inv);
+    const float si = qj * inv3, sj = qi * inv3;
+    if constexpr (WITH_PE) u = fmaT(qi * qj, inv, u);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      a[3 * i + d] = fmaT(r[d], si, a[3 * i + d]);
+      ap[3 * j + d] = fmaT(-r[d], sj, ap[3 * j + d]);
+    }
+  };
   static_for<1, (NA - 1) / 2 + 1>([&](auto O) {
     constexpr int off = decltype(O)::value;
     static_for<0, NA>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      constexpr int j = (i + off) % NA;
-      const float c = pk.template cross<i, j>();
-      float r[3], s;
-      const float inv = pair_term(&xo[3 * i], &xp[3 * j], c, r, s);
-      if constexpr (WITH_PE) u = fmaT(c, inv, u);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
-        ap[3 * j + d] = fmaT(-r[d], s, ap[3 * j + d]);
-      }
+      cross(I, std::integral_constant<int, (i + off) % NA>{});
     });
   });
-  // even NA, offset NA/2: own atoms i < NA/2 both-sided (the partner lane's
-  // i < NA/2 are exactly the other half of these pairs)
   if constexpr (NA % 2 == 0) {
     static_for<0, NA / 2>([&](auto I) {
       constexpr int i = decltype(I)::value;
-      constexpr int j = i + NA / 2;
-      const float c = pk.template cross<i, j>();
-      float r[3], s;
-      const float inv = pair_term(&xo[3 * i], &xp[3 * j], c, r, s);
-      if constexpr (WITH_PE) u = fmaT(c, inv, u);
-#pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
-        ap[3 * j + d] = fmaT(-r[d], s, ap[3 * j + d]);
-      }
+      cross(I, std::integral_constant<int, i + NA / 2>{});
     });
   }
   // offset 0 (atom i with the partner's atom i): both lanes evaluate these, each
   // for its own atom only, with half of the pair's energy
   static_for<0, NA>([&](auto I) {
     constexpr int i = decltype(I)::value;
-    const float c = pk.template cross<i, i>();
-    float r[3], s;
-    const float inv = pair_term(&xo[3 * i], &xp[3 * i], c, r, s);
-    if constexpr (WITH_PE) u = fmaT(0.5f * c, inv, u);
+    const float qi = pk.template q_own<i>(), qp = pk.template q_partner<i>();
+    float r[3], inv;
+    const float inv3 = pair_term<true>(&xo[3 * i], &xo[3 * i], r, inv);
+    const float si = qp * inv3;
+    if constexpr (WITH_PE) u = fmaT(0.5f * (qi * qp), inv, u);
 #pragma unroll
-    for (int d = 0; d < 3; ++d) a[3 * i + d] = fmaT(r[d], s, a[3 * i + d]);
+    for (int d = 0; d < 3; ++d) a[3 * i + d] = fmaT(r[d], si, a[3 * i + d]);
   });
 #pragma unroll
   for (int q = 0; q < NA3; ++q) a[q] += xchg(ap[q]);
@@ -257,7 +275,8 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
     pk.pc0 = mc.pc[0];
     pk.pcd0 = mc.pcd[0];
   }
-  T im[NA];  // 1/m (or c/m, kUniform) of the own atoms
+  if constexpr (CM == kPerSim) pk.kd = p.coulomb_k;
+  T im[NA];  // per-atom factor of the own atoms: c/m (kUniform); k q_i / m_i (per sim)
 #pragma unroll
   for (int i = 0; i < NA; ++i) im[i] = hf ? mc.im[NA + i] : mc.im[i];
   T x[NA3], v[NA3], k[7][NA3];
@@ -412,7 +431,13 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
             v[q] = T(0);
           }
         }
-        if constexpr (CM == kPerSim) pk.load_charges(p.qsim + sg.q_off + loc * N, hf, p.coulomb_k);
+        if constexpr (CM == kPerSim) {
+          pk.load_charges(p.qsim + sg.q_off + loc * N, hf);
+#pragma unroll
+          for (int i = 0; i < NA; ++i)
+            im[i] = static_cast<T>(p.coulomb_k * static_cast<double>(pk.qo[i]) /
+                                   (hf ? mc.md[NA + i] : mc.md[i]));
+        }
         const uint64_t gs = sg.start + loc;
         if (p.resume) {
           steps = p.steps[gs];

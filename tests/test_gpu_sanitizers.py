@@ -28,7 +28,7 @@ def _exe(path):
 
 
 def test_device_paths_under_asan_ubsan():
-    r = run_clean([_exe(os.path.join(SAN, "sanitize_driver")), "device"], timeout=300)
+    r = run_clean([_exe(os.path.join(SAN, "sanitize_driver")), "device"], device=True, timeout=300)
     assert r.returncode == 0, r.stderr[-4000:]
     assert "sanitize driver ok (host and device paths)" in r.stdout
 
@@ -40,7 +40,7 @@ def test_generator_pipeline_under_asan_ubsan(ce, tmp_path, sampler):
     args = ["--sims", str(sims), "--batch", str(batch), "--sampler", sampler]
     out = tmp_path / "san.bin"
     r = run_clean([_exe(os.path.join(PKG_SAN, "generate_dataset")), "--out", str(out)] + args,
-                  timeout=300)
+                  device=True, timeout=300)
     assert r.returncode == 0, r.stderr[-4000:]
     ref = tmp_path / "product.bin"
     r = subprocess.run([os.path.join(ROOT, "coulomb-explosion_amd", "generate_dataset"),

@@ -28,9 +28,17 @@ SANITIZER_REPORT = ("ERROR: AddressSanitizer", "ERROR: LeakSanitizer", "runtime 
                     "WARNING: ThreadSanitizer")
 
 
-def sanitizer_env():
+def sanitizer_env(device=False):
     env = dict(os.environ)
     env["ASAN_OPTIONS"] = "detect_leaks=1:abort_on_error=0:halt_on_error=1"
+    if device:
+        # No quarantine on runs that use the GPU: at process exit the HIP / HSA
+        # runtimes free memory after the device runtime has unloaded, and a
+        # quarantine recycle then hands ASan's device allocator a chunk it can no
+        # longer release (CHECK "!dev_runtime_unloaded_" in
+        # sanitizer_allocator_device.h, inside __cxa_finalize; seen once, with
+        # whatever host chunks the run had left in the quarantine).
+        env["ASAN_OPTIONS"] += ":quarantine_size_mb=0"
     env["LSAN_OPTIONS"] = "suppressions=" + os.path.join(CPP, "lsan_hip.supp")
     env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
     env["TSAN_OPTIONS"] = "halt_on_error=1:exitcode=66"
@@ -43,8 +51,8 @@ def make(target, timeout):
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
 
 
-def run_clean(cmd, **kw):
-    r = subprocess.run(cmd, capture_output=True, text=True, env=sanitizer_env(), **kw)
+def run_clean(cmd, device=False, **kw):
+    r = subprocess.run(cmd, capture_output=True, text=True, env=sanitizer_env(device), **kw)
     for tag in SANITIZER_REPORT:
         assert tag not in r.stderr, r.stderr[-6000:]
     return r
