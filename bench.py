@@ -400,7 +400,9 @@ def run_closure(ce, args) -> dict:
                "median_ensemble_spread": float(np.median(spread))}
     widths = [8 * nat, *su["hidden"], 3 * nat]
     mlp_flops = 2 * sum(a * b for a, b in zip(widths[:-1], widths[1:]))
-    infer_flops = st["candidates"] * mlp_flops * args.members
+    # rows the members actually evaluated (the first two decide most sims, the
+    # others run on the rest: closure_pregate)
+    infer_flops = st["member_rows"] * mlp_flops
     infer_tf = infer_flops / (st["infer_ms"] / 1e3) / 1e12 if st["infer_ms"] else 0.0
     return {
         "n": n,
@@ -412,7 +414,8 @@ def run_closure(ce, args) -> dict:
                          f"stop |dp|/|p| < 1e-7 or t = 1e4): plain DP5(4) to t_cut "
                          f"{args.t_cut:g}, straight-line impulse prior + {args.members}-member MLP "
                          f"ensemble residual, gate {args.gate_tol:g} on the members' per-atom "
-                         f"spread, integration continued elsewhere"),
+                         f"spread (decided from the first two members where their spread "
+                         f"already excludes acceptance), integration continued elsewhere"),
             "sims_per_step": n,
             "method": "closure",
             "plain_sims_per_s": n / times["plain"],
@@ -420,6 +423,7 @@ def run_closure(ce, args) -> dict:
             "closure_over_plain": times["plain"] / times["closure"],
             "fallback_rate": st["fallback"] / max(1, st["candidates"]),
             "candidates": st["candidates"],
+            "member_rows_per_candidate": st["member_rows"] / max(1, st["candidates"]),
             "accepted_by_gate": int(acc.sum()),
             "err_vs_full_integration": {
                 "within_1e-5": float(np.mean(ok)),
@@ -439,8 +443,9 @@ def run_closure(ce, args) -> dict:
                          "final_loss": [float(h[-1]) for h in ens.losses]},
         },
         "roofline": {
-            "bound": "mfma", "kernel": "fused ensemble inference (mlp_fused_forward_nout), "
-                                       "with features and gate",
+            "bound": "mfma", "kernel": "fused ensemble inference (mlp_fused_forward), "
+                                       "with features and gate; FLOP of the member rows "
+                                       "evaluated",
             "achieved": infer_tf, "peak": 2500.0, "unit": "TFLOP/s", "frac": infer_tf / 2500.0,
             "traffic": None, "flops_per_sim_per_member": mlp_flops},
     }

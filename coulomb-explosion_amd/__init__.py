@@ -88,7 +88,7 @@ class _ClosureStats(C.Structure):
     _fields_ = [("sims", C.c_uint64), ("candidates", C.c_uint64), ("fallback", C.c_uint64),
                 ("cut_attempts", C.c_uint64), ("fallback_attempts", C.c_uint64),
                 ("cut_ms", C.c_double), ("infer_ms", C.c_double), ("fallback_ms", C.c_double),
-                ("total_ms", C.c_double)]
+                ("total_ms", C.c_double), ("member_rows", C.c_uint64)]
 
 
 class _LaunchConfig(C.Structure):

@@ -159,7 +159,11 @@ struct GateArgs {
   double* mom;              // [sim][n][3]
   float* mom32;
   uint32_t* list;           // fallback sims
-  uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut
+  uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut, [2] survivors
+  // Compacted rows (the survivors of closure_pregate): entry r of y is chunk row
+  // rows[r] (prior, status and outputs of that row); nullptr = the chunk itself.
+  const uint32_t* rows;
+  int32_t count_cands;      // add the candidates to counts[1] (not when the pre-gate has)
 };
 
 // Block-aggregated append (256 threads): one atomic per block on each counter
@@ -195,7 +199,8 @@ __device__ __forceinline__ void block_append(bool cand, bool flag, uint32_t valu
 template <int N>
 __global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  const uint64_t g = a.first + r;
+  const uint64_t rr = a.rows != nullptr && r < a.count ? a.rows[r] : r;  // chunk row
+  const uint64_t g = a.first + rr;
   const bool cand = r < a.count && a.status[g] == ce::kSimOkTmax;
   bool ok = false;
   if (cand) {
@@ -228,7 +233,7 @@ __global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int j = 3 * i + c;
-        pm[j] = a.prior[r * 3 * N + j] + pm[j] * a.out_scale[j];  // predicted momentum
+        pm[j] = a.prior[rr * 3 * N + j] + pm[j] * a.out_scale[j];  // predicted momentum
         p2 += pm[j] * pm[j];
       }
       ok = ok && (var[i] * inv_m1 <= a.tol2 * p2);  // false for NaN
@@ -242,7 +247,115 @@ __global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
       a.status[g] = ce::kSimClosure;
     }
   }
-  block_append(cand, cand && !ok, static_cast<uint32_t>(g), a.counts, a.list);
+  block_append(a.count_cands ? cand : false, cand && !ok, static_cast<uint32_t>(g), a.counts,
+               a.list);
+}
+
+// Early rejection from the first two members.  The gate accepts a sim when for
+// every atom D / (M - 1) <= tol^2 |P + zbar|^2, where z_k are the members' scaled
+// outputs, zbar their mean, D = sum_k |z_k - zbar|^2 and P the prior.  With the
+// first two members' mean z12, D12 = |z_1 - z_2|^2 / 2 and u = |zbar - z12|, the
+// parallel-axis identity and the least spread of the other M - 2 members for a
+// given mean (all equal) give D >= D12 + k u^2, k = 2M / (M - 2); and
+// |P + zbar| <= A + u with A = |P + z12|.  So with c = (M - 1) tol^2 < k,
+// acceptance needs D12 <= max_u [c (A + u)^2 - k u^2] = c k A^2 / (k - c), a bound
+// the other members attain (all at one point along P + z12).  A sim past it on
+// some atom is rejected here, without the other members (at the 1e-5 gate:
+// nearly every sim); the others go to a compacted list for the remaining members
+// and the full gate.  A relative margin of 1e-9 keeps rounding (FP64 here) on
+// the side of the full gate.
+struct PregateArgs {
+  const double* prior;      // [count][3n]
+  const float* y;           // members 0 and 1: [2][count][3n] (member stride count 3n)
+  const float* out_scale;   // [3n]
+  uint64_t first, count;
+  double c;                 // (members - 1) tol^2, < 1
+  double k;                 // 2 members / (members - 2)
+  uint8_t* status;
+  uint32_t* list;           // fallback sims
+  uint32_t* rows;           // survivors: chunk rows
+  uint32_t* counts;         // [0] fallback, [1] candidates, [2] survivors
+};
+
+template <int N>
+__global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
+  constexpr int O = 3 * N;
+  const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+  const uint64_t g = a.first + r;
+  const bool cand = r < a.count && a.status[g] == ce::kSimOkTmax;
+  bool reject = false;
+  if (cand) {
+    const float* y0 = a.y + r * O;
+    const float* y1 = y0 + a.count * O;
+    const double bound = a.c * a.k / (a.k - a.c) * (1.0 + 1e-9);
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      double d12 = 0.0, a2 = 0.0;
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const int j = 3 * i + cc;
+        const double z0 = static_cast<double>(y0[j]) * a.out_scale[j];
+        const double z1 = static_cast<double>(y1[j]) * a.out_scale[j];
+        const double d = z0 - z1;
+        const double m = a.prior[r * O + j] + 0.5 * (z0 + z1);
+        d12 += 0.5 * d * d;
+        a2 += m * m;
+      }
+      reject = reject || d12 > bound * a2;  // false for NaN: left to the full gate
+    }
+  }
+  // two lists: the rejected sims (fallback) and the surviving rows
+  __shared__ uint32_t wn[4][3];
+  __shared__ uint32_t base_f, base_s;
+  const int lane = __lane_id(), wave = threadIdx.x >> 6;
+  const bool surv = cand && !reject, rej = cand && reject;
+  const uint64_t fm = __ballot(rej), sm = __ballot(surv), cm = __ballot(cand);
+  if (lane == 0) {
+    wn[wave][0] = static_cast<uint32_t>(__popcll(fm));
+    wn[wave][1] = static_cast<uint32_t>(__popcll(sm));
+    wn[wave][2] = static_cast<uint32_t>(__popcll(cm));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t nf = 0, ns = 0, nc = 0;
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t f = wn[w][0], sv = wn[w][1];
+      wn[w][0] = nf;
+      wn[w][1] = ns;
+      nf += f;
+      ns += sv;
+      nc += wn[w][2];
+    }
+    base_f = nf ? atomicAdd(a.counts, nf) : 0u;
+    base_s = ns ? atomicAdd(a.counts + 2, ns) : 0u;
+    if (nc) atomicAdd(a.counts + 1, nc);
+  }
+  __syncthreads();
+  const uint64_t below = (1ull << lane) - 1;
+  if (rej) a.list[base_f + wn[wave][0] + __popcll(fm & below)] = static_cast<uint32_t>(g);
+  if (surv) a.rows[base_s + wn[wave][1] + __popcll(sm & below)] = static_cast<uint32_t>(r);
+}
+
+// The survivors' features and first two members' outputs, compacted.
+__global__ void closure_gather(const uint32_t* rows, const uint32_t* n_rows, int fw, int o,
+                               uint64_t count, const float* feat, const float* y, float* featc,
+                               float* yc) {
+  const uint64_t n = *n_rows;
+  const uint64_t per = static_cast<uint64_t>(fw) + 2 * o;
+  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < n * per;
+       e += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t s = e / per, k = e % per, r = rows[s];
+    if (k < static_cast<uint64_t>(fw)) {
+      featc[s * fw + k] = feat[r * fw + k];
+    } else {
+      const uint64_t m = (k - fw) / o, j = (k - fw) % o;  // member 0 or 1
+      yc[m * n * o + s * o + j] = y[m * count * o + r * o + j];
+    }
+  }
+}
+
+__global__ void zero_u32(uint32_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
 __global__ void zero_counts(uint32_t* p) {
@@ -264,7 +377,10 @@ struct ce_closure {
   float* feat = nullptr;
   double* prior = nullptr;
   float* y = nullptr;          // [kMaxMembers][chunk][3n]
-  uint32_t* counts = nullptr;  // fallback queued, cut-stopped sims
+  uint32_t* counts = nullptr;  // fallback queued, cut-stopped sims, pre-gate survivors
+  uint32_t* rows = nullptr;    // pre-gate survivors: chunk rows [chunk]
+  float* featc = nullptr;      // their features [chunk][8n]
+  float* yc = nullptr;         // their member outputs [kMaxMembers][survivors][3n]
   hipEvent_t ev[4] = {};
   ce_closure_stats stats{};
 };
@@ -292,7 +408,8 @@ int dalloc(T** p, int64_t n) {
 void free_closure(ce_closure* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  void* b[] = {c->shift, c->inv_scale, c->out_scale, c->feat, c->prior, c->y, c->counts};
+  void* b[] = {c->shift, c->inv_scale, c->out_scale, c->feat, c->prior, c->y, c->counts,
+               c->rows, c->featc, c->yc};
   for (void* p : b)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev)
@@ -302,16 +419,19 @@ void free_closure(ce_closure* c) {
 
 int ensure_chunk(ce_closure* c, int64_t n) {
   if (n <= c->chunk_cap) return CE_OK;
-  void* b[] = {c->feat, c->prior, c->y};
+  void* b[] = {c->feat, c->prior, c->y, c->rows, c->featc, c->yc};
   for (void* p : b)
     if (p) CL_HIP(hipFree(p));
-  c->feat = nullptr;
+  c->feat = c->featc = c->y = c->yc = nullptr;
   c->prior = nullptr;
-  c->y = nullptr;
+  c->rows = nullptr;
   c->chunk_cap = 0;
   int rc = dalloc(&c->feat, n * 8 * c->n);
   if (rc == CE_OK) rc = dalloc(&c->prior, n * 3 * c->n);
   if (rc == CE_OK) rc = dalloc(&c->y, int64_t(kMaxMembers) * n * 3 * c->n);
+  if (rc == CE_OK) rc = dalloc(&c->rows, n);
+  if (rc == CE_OK) rc = dalloc(&c->featc, n * 8 * c->n);
+  if (rc == CE_OK) rc = dalloc(&c->yc, int64_t(kMaxMembers) * n * 3 * c->n);
   if (rc == CE_OK) c->chunk_cap = n;
   return rc;
 }
@@ -345,6 +465,21 @@ int launch_gate(ce_closure* c, const GateArgs& a, hipStream_t s) {
     break;
     CL_G(1) CL_G(2) CL_G(3) CL_G(4) CL_G(5) CL_G(6) CL_G(7) CL_G(8) CL_G(9) CL_G(10)
 #undef CL_G
+    default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
+  }
+  CL_HIP(hipGetLastError());
+  return CE_OK;
+}
+
+int launch_pregate(ce_closure* c, const PregateArgs& a, hipStream_t s) {
+  ce::clear_stale_error();
+  switch (c->n) {
+#define CL_P(NN)                                                                             \
+  case NN:                                                                                   \
+    hipLaunchKernelGGL(closure_pregate<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a);      \
+    break;
+    CL_P(1) CL_P(2) CL_P(3) CL_P(4) CL_P(5) CL_P(6) CL_P(7) CL_P(8) CL_P(9) CL_P(10)
+#undef CL_P
     default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
   }
   CL_HIP(hipGetLastError());
@@ -402,7 +537,7 @@ int ce_closure_create(int32_t n_atoms, const double* masses, const double* charg
   if (rc == CE_OK) rc = dalloc(&c->shift, 8 * n_atoms);
   if (rc == CE_OK) rc = dalloc(&c->inv_scale, 8 * n_atoms);
   if (rc == CE_OK) rc = dalloc(&c->out_scale, 3 * n_atoms);
-  if (rc == CE_OK) rc = dalloc(&c->counts, 2);
+  if (rc == CE_OK) rc = dalloc(&c->counts, 3);
   for (int i = 0; i < 4 && rc == CE_OK; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CE_ERR_HIP, "hipEventCreate");
   if (rc != CE_OK) {
@@ -521,10 +656,15 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
   rc = ensure_chunk(c, std::min<int64_t>(chunk, std::max<int64_t>(1, n)));
   if (rc != CE_OK) return rc;
   const int o = 3 * c->n;
+  // With three or more members and a gate that is not wide open, the first two
+  // members decide most sims (closure_pregate); the others run on the survivors.
+  const double c_bound = (M - 1) * gate_tol * gate_tol;
+  const bool pregate = M >= 3 && c_bound < 1.0;
   for (uint64_t f0 = 0; f0 < n && rc == CE_OK; f0 += chunk) {
     const uint64_t cnt = std::min<uint64_t>(chunk, n - f0);
     rc = launch_features(c, feature_args(c, e, f0, cnt, 0, true), s);
-    for (int k = 0; k < M && rc == CE_OK; ++k)
+    const int first_pass = pregate ? 2 : M;
+    for (int k = 0; k < first_pass && rc == CE_OK; ++k)
       rc = ce::mlp_forward_device(c->members[k], c->feat, static_cast<int64_t>(cnt),
                                   c->y + static_cast<int64_t>(k) * cnt * o, s);
     if (rc != CE_OK) break;
@@ -541,6 +681,45 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
     g.mom32 = e->mom32;
     g.list = e->list;
     g.counts = c->counts;
+    g.rows = nullptr;
+    g.count_cands = 1;
+    if (pregate) {
+      hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, c->counts + 2);
+      PregateArgs pa{};
+      pa.prior = c->prior;
+      pa.y = c->y;
+      pa.out_scale = c->out_scale;
+      pa.first = f0;
+      pa.count = cnt;
+      pa.c = c_bound;
+      pa.k = 2.0 * M / (M - 2);
+      pa.status = e->status;
+      pa.list = e->list;
+      pa.rows = c->rows;
+      pa.counts = c->counts;
+      rc = launch_pregate(c, pa, s);
+      if (rc != CE_OK) break;
+      uint32_t n_surv = 0;
+      CL_HIP(hipMemcpyAsync(&n_surv, c->counts + 2, sizeof(n_surv), hipMemcpyDeviceToHost, s));
+      CL_HIP(hipStreamSynchronize(s));
+      st.member_rows += 2 * cnt + static_cast<uint64_t>(M - 2) * n_surv;
+      if (n_surv == 0) continue;
+      const int fw = 8 * c->n;
+      hipLaunchKernelGGL(closure_gather, dim3(std::min<uint64_t>(8192, (n_surv + 255) / 256 * 4)),
+                         dim3(256), 0, s, c->rows, c->counts + 2, fw, o, cnt, c->feat, c->y,
+                         c->featc, c->yc);
+      CL_HIP(hipGetLastError());
+      for (int k = 2; k < M && rc == CE_OK; ++k)
+        rc = ce::mlp_forward_device(c->members[k], c->featc, static_cast<int64_t>(n_surv),
+                                    c->yc + static_cast<int64_t>(k) * n_surv * o, s);
+      if (rc != CE_OK) break;
+      g.y = c->yc;
+      g.count = n_surv;
+      g.rows = c->rows;
+      g.count_cands = 0;  // counted by the pre-gate
+    } else {
+      st.member_rows += static_cast<uint64_t>(M) * cnt;
+    }
     rc = launch_gate(c, g, s);
   }
   if (rc != CE_OK) return rc;

@@ -377,6 +377,10 @@ typedef struct {
   double infer_ms;            /* features + ensemble + gate */
   double fallback_ms;         /* continuation kernel */
   double total_ms;            /* whole ce_closure_run on the device stream */
+  uint64_t member_rows;       /* rows the members evaluated, summed over members: with
+                                 three or more members the first two decide most sims
+                                 (an exact early-rejection bound) and the others run on
+                                 the rest only */
 } ce_closure_stats;
 int ce_closure_create(int32_t n_atoms, const double* masses, const double* charges,
                       double coulomb_k, ce_closure** out);

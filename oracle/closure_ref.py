@@ -4,7 +4,9 @@ The closure has no counterpart in the reference.  Its prior is the remaining
 Coulomb impulse along straight-line trajectories from the cut on,
     J_ij = k q_i q_j int_0^inf r(t)/|r(t)|^3 dt,  r(t) = r + u t,
 whose closed form (r/|r| + u/|u|) / (|r||u| + r.u) is checked here against
-numerical quadrature, followed by a rescale of the kinetic energy to E0.
+numerical quadrature, followed by a rescale of the kinetic energy to E0.  Also the
+ensemble gate and its early rejection from two members (closure.hip,
+closure_gate / closure_pregate).
 """
 from __future__ import annotations
 
@@ -55,3 +57,27 @@ def features(x, v, masses, charges):
     cm = (m[:, None] * x).sum(0) / m.sum()
     per_atom = np.concatenate([x - cm, m[:, None] * v], axis=1).reshape(-1)
     return np.concatenate([per_atom, np.asarray(charges, np.float64), m])
+
+
+def gate_accepts(z, prior, tol):
+    """The ensemble gate: z [M][n][3] scaled member outputs, prior [n][3].  Accept
+    when for every atom sum_k |z_k - zbar|^2 / (M - 1) <= tol^2 |prior + zbar|^2."""
+    z = np.asarray(z, dtype=np.float64)
+    m = z.shape[0]
+    zbar = z.mean(axis=0)
+    d = ((z - zbar) ** 2).sum(axis=(0, 2))
+    p2 = ((prior + zbar) ** 2).sum(axis=1)
+    return bool(np.all(d / (m - 1) <= tol * tol * p2))
+
+
+def pregate_rejects(z1, z2, prior, members, tol):
+    """Early rejection from the first two of M = `members` outputs: with
+    c = (M-1) tol^2, k = 2M/(M-2), D12 = |z1 - z2|^2 / 2 and A = |prior + (z1+z2)/2|
+    per atom, the gate cannot accept whatever the other members output when
+    D12 > c k A^2 / (k - c) on some atom (derivation in closure.hip)."""
+    c = (members - 1) * tol * tol
+    k = 2.0 * members / (members - 2)
+    z12 = 0.5 * (np.asarray(z1, np.float64) + np.asarray(z2, np.float64))
+    d12 = 0.5 * ((np.asarray(z1, np.float64) - z2) ** 2).sum(axis=1)
+    a2 = ((prior + z12) ** 2).sum(axis=1)
+    return bool(np.any(d12 > c * k / (k - c) * a2))

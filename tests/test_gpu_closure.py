@@ -223,7 +223,18 @@ def test_closure_gate_follows_spread(gpu, trained):
     acc = res.status == ce.SIM_CLOSURE
     far = np.abs(rel - tol) > 1e-3 * tol  # away from the threshold (rounding)
     np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
-    assert clo.stats()["fallback"] == np.count_nonzero(~acc)
+    st = clo.stats()
+    assert st["fallback"] == np.count_nonzero(~acc)
+    # the first two members decided part of the sims (early rejection), the third
+    # member ran on the rest, and the decisions above are still the full gate's
+    m, cand = len(ens.members), st["candidates"]
+    assert 2 * cand < st["member_rows"] < m * cand
+    # at the closed gate the first two members decide every sim that they disagree on;
+    # wide open, every member runs on every sim
+    clo.run(eng, cfg, T_CUT, 0.0)
+    assert clo.stats()["member_rows"] == 2 * cand
+    clo.run(eng, cfg, T_CUT, 1e30)
+    assert clo.stats()["member_rows"] == m * cand
     assert np.all(res.status[~acc] == ce.SIM_OK_TMAX) and np.all(res.t_final[~acc] == 1e4)
 
 
