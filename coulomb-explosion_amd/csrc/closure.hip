@@ -148,9 +148,21 @@ __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs
   for (uint64_t e = threadIdx.x; e < rows * 3 * N; e += kFeatBlock) a.prior[r0 * 3 * N + e] = psf[e];
 }
 
+// The counters, 256 bytes apart (atomics on one line serialise with each other):
+// fallback sims queued, sims stopped by the cut (candidates), pre-gate
+// survivors of the current chunk and of all chunks.
+constexpr int kFallback = 0, kCands = 64, kSurv = 128, kSurvTotal = 192, kCounters = 256;
+// Threads per block of the gate and the pre-gate: every block makes one atomic
+// add per list, so larger blocks make fewer of them.  The gate holds 4N doubles
+// per sim: past six atoms it takes half the block, to keep its registers.
+constexpr int kGateBlock = 1024;
+template <int N>
+constexpr int gate_block() { return N <= 6 ? kGateBlock : kGateBlock / 2; }
+
 struct GateArgs {
   const double* prior;      // [count][3n]
-  const float* y;           // [members][count][3n]
+  const float* y;           // members [0, split): [split][stride][3n] at chunk rows
+  const float* y2;          // members [split, members): [..][stride][3n] at compact rows
   const float* out_scale;   // [3n]
   int members;
   uint64_t first, count;
@@ -159,20 +171,25 @@ struct GateArgs {
   double* mom;              // [sim][n][3]
   float* mom32;
   uint32_t* list;           // fallback sims
-  uint32_t* counts;         // [0] fallback sims queued, [1] sims stopped by the cut, [2] survivors
-  // Compacted rows (the survivors of closure_pregate): entry r of y is chunk row
-  // rows[r] (prior, status and outputs of that row); nullptr = the chunk itself.
+  uint32_t* counts;         // kFallback, kCands
+  // Compacted rows (the survivors of closure_pregate): row r of the gate is chunk
+  // row rows[r] (prior, status, members [0, split)); nullptr = the chunk itself.
   const uint32_t* rows;
-  int32_t count_cands;      // add the candidates to counts[1] (not when the pre-gate has)
+  const uint32_t* count_dev;  // or nullptr: the rows are min(count, *count_dev), read here
+  uint64_t stride;          // rows between members in y and y2
+  int32_t split;
+  int32_t count_cands;      // add the candidates to kCands (not when the pre-gate has)
 };
 
-// Block-aggregated append (256 threads): one atomic per block on each counter
-// (the fallback list and the candidate count), each flagged lane gets its slot.
-// Per-wave atomics on two addresses serialise at the memory side when every sim
-// falls back (two per 64 sims).
+// Block-aggregated append (B threads): one atomic per block on each counter (the
+// fallback list and the candidate count), each flagged lane gets its slot.
+// Atomics on one address serialise at the memory side: per-wave ones would bound
+// the kernel when every sim falls back.
+template <int B>
 __device__ __forceinline__ void block_append(bool cand, bool flag, uint32_t value, uint32_t* counts,
                                              uint32_t* list) {
-  __shared__ uint32_t wave_n[4][2];
+  constexpr int W = B / 64;
+  __shared__ uint32_t wave_n[W][2];
   __shared__ uint32_t base_s;
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const uint64_t fmask = __ballot(flag), cmask = __ballot(cand);
@@ -183,49 +200,56 @@ __device__ __forceinline__ void block_append(bool cand, bool flag, uint32_t valu
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t nf = 0, nc = 0;
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < W; ++w) {
       const uint32_t f = wave_n[w][0];
       wave_n[w][0] = nf;  // exclusive prefix: the wave's offset in the block's slots
       nf += f;
       nc += wave_n[w][1];
     }
-    base_s = nf ? atomicAdd(counts, nf) : 0u;
-    if (nc) atomicAdd(counts + 1, nc);
+    base_s = nf ? atomicAdd(counts + kFallback, nf) : 0u;
+    if (nc) atomicAdd(counts + kCands, nc);
   }
   __syncthreads();
   if (flag) list[base_s + wave_n[wave][0] + __popcll(fmask & ((1ull << lane) - 1))] = value;
 }
 
 template <int N>
-__global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
+__global__ void __launch_bounds__(gate_block<N>()) closure_gate(const GateArgs a) {
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-  const uint64_t rr = a.rows != nullptr && r < a.count ? a.rows[r] : r;  // chunk row
+  uint64_t count = a.count;
+  if (a.count_dev != nullptr) count = min(count, static_cast<uint64_t>(*a.count_dev));
+  const uint64_t rr = a.rows != nullptr && r < count ? a.rows[r] : r;  // chunk row
   const uint64_t g = a.first + rr;
-  const bool cand = r < a.count && a.status[g] == ce::kSimOkTmax;
+  const bool cand = r < count && a.status[g] == ce::kSimOkTmax;
   bool ok = false;
   if (cand) {
     // member-major passes: the 3N loads of a member are independent of each other
-    const uint64_t cs = a.count * 3 * N;  // member stride
-    const float* yr = a.y + r * 3 * N;
+    const uint64_t cs = a.stride * 3 * N;  // member stride
+    const float* yr = a.y + rr * 3 * N;
+    const float* yr2 = a.y2 + r * 3 * N - a.split * cs;  // member k >= split at yr2 + k cs
     const double inv_m = a.members > 0 ? 1.0 / a.members : 0.0;
     const double inv_m1 = a.members > 1 ? 1.0 / (a.members - 1) : 1.0;
     double pm[3 * N];
 #pragma unroll
     for (int j = 0; j < 3 * N; ++j) pm[j] = 0.0;
-    for (int k = 0; k < a.members; ++k)
+    for (int k = 0; k < a.members; ++k) {
+      const float* yk = (k < a.split ? yr : yr2) + k * cs;
 #pragma unroll
-      for (int j = 0; j < 3 * N; ++j) pm[j] += yr[k * cs + j];
+      for (int j = 0; j < 3 * N; ++j) pm[j] += yk[j];
+    }
 #pragma unroll
     for (int j = 0; j < 3 * N; ++j) pm[j] *= inv_m;
     double var[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) var[i] = 0.0;
-    for (int k = 0; k < a.members; ++k)
+    for (int k = 0; k < a.members; ++k) {
+      const float* yk = (k < a.split ? yr : yr2) + k * cs;
 #pragma unroll
       for (int j = 0; j < 3 * N; ++j) {
-        const double d = (yr[k * cs + j] - pm[j]) * a.out_scale[j];
+        const double d = (yk[j] - pm[j]) * a.out_scale[j];
         var[j / 3] += d * d;
       }
+    }
     ok = a.members >= 2;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -247,7 +271,7 @@ __global__ void __launch_bounds__(256) closure_gate(const GateArgs a) {
       a.status[g] = ce::kSimClosure;
     }
   }
-  block_append(a.count_cands ? cand : false, cand && !ok, static_cast<uint32_t>(g), a.counts,
+  block_append<gate_block<N>()>(a.count_cands ? cand : false, cand && !ok, static_cast<uint32_t>(g), a.counts,
                a.list);
 }
 
@@ -274,11 +298,11 @@ struct PregateArgs {
   uint8_t* status;
   uint32_t* list;           // fallback sims
   uint32_t* rows;           // survivors: chunk rows
-  uint32_t* counts;         // [0] fallback, [1] candidates, [2] survivors
+  uint32_t* counts;         // kFallback, kCands, kSurv
 };
 
 template <int N>
-__global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
+__global__ void __launch_bounds__(kGateBlock) closure_pregate(const PregateArgs a) {
   constexpr int O = 3 * N;
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   const uint64_t g = a.first + r;
@@ -305,7 +329,8 @@ __global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
     }
   }
   // two lists: the rejected sims (fallback) and the surviving rows
-  __shared__ uint32_t wn[4][3];
+  constexpr int W = kGateBlock / 64;
+  __shared__ uint32_t wn[W][3];
   __shared__ uint32_t base_f, base_s;
   const int lane = __lane_id(), wave = threadIdx.x >> 6;
   const bool surv = cand && !reject, rej = cand && reject;
@@ -318,7 +343,7 @@ __global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t nf = 0, ns = 0, nc = 0;
-    for (int w = 0; w < 4; ++w) {
+    for (int w = 0; w < W; ++w) {
       const uint32_t f = wn[w][0], sv = wn[w][1];
       wn[w][0] = nf;
       wn[w][1] = ns;
@@ -326,9 +351,9 @@ __global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
       ns += sv;
       nc += wn[w][2];
     }
-    base_f = nf ? atomicAdd(a.counts, nf) : 0u;
-    base_s = ns ? atomicAdd(a.counts + 2, ns) : 0u;
-    if (nc) atomicAdd(a.counts + 1, nc);
+    base_f = nf ? atomicAdd(a.counts + kFallback, nf) : 0u;
+    base_s = ns ? atomicAdd(a.counts + kSurv, ns) : 0u;
+    if (nc) atomicAdd(a.counts + kCands, nc);
   }
   __syncthreads();
   const uint64_t below = (1ull << lane) - 1;
@@ -336,30 +361,17 @@ __global__ void __launch_bounds__(256) closure_pregate(const PregateArgs a) {
   if (surv) a.rows[base_s + wn[wave][1] + __popcll(sm & below)] = static_cast<uint32_t>(r);
 }
 
-// The survivors' features and first two members' outputs, compacted.
-__global__ void closure_gather(const uint32_t* rows, const uint32_t* n_rows, int fw, int o,
-                               uint64_t count, const float* feat, const float* y, float* featc,
-                               float* yc) {
-  const uint64_t n = *n_rows;
-  const uint64_t per = static_cast<uint64_t>(fw) + 2 * o;
-  for (uint64_t e = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; e < n * per;
-       e += uint64_t(gridDim.x) * blockDim.x) {
-    const uint64_t s = e / per, k = e % per, r = rows[s];
-    if (k < static_cast<uint64_t>(fw)) {
-      featc[s * fw + k] = feat[r * fw + k];
-    } else {
-      const uint64_t m = (k - fw) / o, j = (k - fw) % o;  // member 0 or 1
-      yc[m * n * o + s * o + j] = y[m * count * o + r * o + j];
-    }
-  }
-}
-
 __global__ void zero_u32(uint32_t* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
 __global__ void zero_counts(uint32_t* p) {
-  if (threadIdx.x < 2 && blockIdx.x == 0) p[threadIdx.x] = 0u;
+  if (threadIdx.x < kCounters && blockIdx.x == 0) p[threadIdx.x] = 0u;
+}
+
+// Total of the chunks' survivors, read once after the last chunk.
+__global__ void add_u32(uint32_t* total, const uint32_t* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *total += *p;
 }
 
 }  // namespace
@@ -377,10 +389,9 @@ struct ce_closure {
   float* feat = nullptr;
   double* prior = nullptr;
   float* y = nullptr;          // [kMaxMembers][chunk][3n]
-  uint32_t* counts = nullptr;  // fallback queued, cut-stopped sims, pre-gate survivors
+  uint32_t* counts = nullptr;  // [kCounters]: kFallback, kCands, kSurv, kSurvTotal
   uint32_t* rows = nullptr;    // pre-gate survivors: chunk rows [chunk]
-  float* featc = nullptr;      // their features [chunk][8n]
-  float* yc = nullptr;         // their member outputs [kMaxMembers][survivors][3n]
+  float* yc = nullptr;         // members 2.. on those rows [kMaxMembers - 2][chunk][3n]
   hipEvent_t ev[4] = {};
   ce_closure_stats stats{};
 };
@@ -409,7 +420,7 @@ void free_closure(ce_closure* c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   void* b[] = {c->shift, c->inv_scale, c->out_scale, c->feat, c->prior, c->y, c->counts,
-               c->rows, c->featc, c->yc};
+               c->rows, c->yc};
   for (void* p : b)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : c->ev)
@@ -419,10 +430,10 @@ void free_closure(ce_closure* c) {
 
 int ensure_chunk(ce_closure* c, int64_t n) {
   if (n <= c->chunk_cap) return CE_OK;
-  void* b[] = {c->feat, c->prior, c->y, c->rows, c->featc, c->yc};
+  void* b[] = {c->feat, c->prior, c->y, c->rows, c->yc};
   for (void* p : b)
     if (p) CL_HIP(hipFree(p));
-  c->feat = c->featc = c->y = c->yc = nullptr;
+  c->feat = c->y = c->yc = nullptr;
   c->prior = nullptr;
   c->rows = nullptr;
   c->chunk_cap = 0;
@@ -430,13 +441,13 @@ int ensure_chunk(ce_closure* c, int64_t n) {
   if (rc == CE_OK) rc = dalloc(&c->prior, n * 3 * c->n);
   if (rc == CE_OK) rc = dalloc(&c->y, int64_t(kMaxMembers) * n * 3 * c->n);
   if (rc == CE_OK) rc = dalloc(&c->rows, n);
-  if (rc == CE_OK) rc = dalloc(&c->featc, n * 8 * c->n);
-  if (rc == CE_OK) rc = dalloc(&c->yc, int64_t(kMaxMembers) * n * 3 * c->n);
+  if (rc == CE_OK) rc = dalloc(&c->yc, int64_t(kMaxMembers - 2) * n * 3 * c->n);
   if (rc == CE_OK) c->chunk_cap = n;
   return rc;
 }
 
 unsigned blocks(uint64_t n) { return static_cast<unsigned>((n + 255) / 256); }
+unsigned gate_blocks(uint64_t n, int b) { return static_cast<unsigned>((n + b - 1) / b); }
 
 int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
   ce::clear_stale_error();
@@ -461,7 +472,8 @@ int launch_gate(ce_closure* c, const GateArgs& a, hipStream_t s) {
   switch (c->n) {
 #define CL_G(NN)                                                                          \
   case NN:                                                                                \
-    hipLaunchKernelGGL(closure_gate<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a); \
+    hipLaunchKernelGGL(closure_gate<NN>, dim3(gate_blocks(a.count, gate_block<NN>())),        \
+                       dim3(gate_block<NN>()), 0, s, a);                                      \
     break;
     CL_G(1) CL_G(2) CL_G(3) CL_G(4) CL_G(5) CL_G(6) CL_G(7) CL_G(8) CL_G(9) CL_G(10)
 #undef CL_G
@@ -476,7 +488,8 @@ int launch_pregate(ce_closure* c, const PregateArgs& a, hipStream_t s) {
   switch (c->n) {
 #define CL_P(NN)                                                                             \
   case NN:                                                                                   \
-    hipLaunchKernelGGL(closure_pregate<NN>, dim3(blocks(a.count)), dim3(256), 0, s, a);      \
+    hipLaunchKernelGGL(closure_pregate<NN>, dim3(gate_blocks(a.count, kGateBlock)),        \
+                       dim3(kGateBlock), 0, s, a);                                            \
     break;
     CL_P(1) CL_P(2) CL_P(3) CL_P(4) CL_P(5) CL_P(6) CL_P(7) CL_P(8) CL_P(9) CL_P(10)
 #undef CL_P
@@ -537,7 +550,7 @@ int ce_closure_create(int32_t n_atoms, const double* masses, const double* charg
   if (rc == CE_OK) rc = dalloc(&c->shift, 8 * n_atoms);
   if (rc == CE_OK) rc = dalloc(&c->inv_scale, 8 * n_atoms);
   if (rc == CE_OK) rc = dalloc(&c->out_scale, 3 * n_atoms);
-  if (rc == CE_OK) rc = dalloc(&c->counts, 3);
+  if (rc == CE_OK) rc = dalloc(&c->counts, kCounters);
   for (int i = 0; i < 4 && rc == CE_OK; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = fail(CE_ERR_HIP, "hipEventCreate");
   if (rc != CE_OK) {
@@ -650,16 +663,20 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
   // 2. features, ensemble and gate, chunk by chunk
   CL_HIP(hipEventRecord(c->ev[1], s));
   ce::clear_stale_error();  // zero_counts is checked with the feature launch
-  hipLaunchKernelGGL(zero_counts, dim3(1), dim3(64), 0, s, c->counts);
+  hipLaunchKernelGGL(zero_counts, dim3(1), dim3(kCounters), 0, s, c->counts);
   const int M = static_cast<int>(c->members.size());
   const int64_t chunk = 1 << 20;
   rc = ensure_chunk(c, std::min<int64_t>(chunk, std::max<int64_t>(1, n)));
   if (rc != CE_OK) return rc;
   const int o = 3 * c->n;
-  // With three or more members and a gate that is not wide open, the first two
-  // members decide most sims (closure_pregate); the others run on the survivors.
+  // With three or more members, a gate that is not wide open and the fused
+  // inference kernel, the first two members decide most sims (closure_pregate);
+  // the others run on the survivors, whose count stays on the device: the member
+  // passes read their rows through the survivor list, the gate reads the first
+  // two members at the chunk rows and the others at the compact rows.
   const double c_bound = (M - 1) * gate_tol * gate_tol;
-  const bool pregate = M >= 3 && c_bound < 1.0;
+  bool pregate = M >= 3 && c_bound < 1.0;
+  for (int k = 2; k < M; ++k) pregate = pregate && ce::mlp_fused_ready(c->members[k]);
   for (uint64_t f0 = 0; f0 < n && rc == CE_OK; f0 += chunk) {
     const uint64_t cnt = std::min<uint64_t>(chunk, n - f0);
     rc = launch_features(c, feature_args(c, e, f0, cnt, 0, true), s);
@@ -675,6 +692,8 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
     g.members = M;
     g.first = f0;
     g.count = cnt;
+    g.stride = cnt;
+    g.split = M;
     g.tol2 = gate_tol * gate_tol;
     g.status = e->status;
     g.mom = e->mom;
@@ -683,8 +702,9 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
     g.counts = c->counts;
     g.rows = nullptr;
     g.count_cands = 1;
+    st.member_rows += static_cast<uint64_t>(first_pass) * cnt;
     if (pregate) {
-      hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, c->counts + 2);
+      hipLaunchKernelGGL(zero_u32, dim3(1), dim3(64), 0, s, c->counts + kSurv);
       PregateArgs pa{};
       pa.prior = c->prior;
       pa.y = c->y;
@@ -699,37 +719,31 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
       pa.counts = c->counts;
       rc = launch_pregate(c, pa, s);
       if (rc != CE_OK) break;
-      uint32_t n_surv = 0;
-      CL_HIP(hipMemcpyAsync(&n_surv, c->counts + 2, sizeof(n_surv), hipMemcpyDeviceToHost, s));
-      CL_HIP(hipStreamSynchronize(s));
-      st.member_rows += 2 * cnt + static_cast<uint64_t>(M - 2) * n_surv;
-      if (n_surv == 0) continue;
-      const int fw = 8 * c->n;
-      hipLaunchKernelGGL(closure_gather, dim3(std::min<uint64_t>(8192, (n_surv + 255) / 256 * 4)),
-                         dim3(256), 0, s, c->rows, c->counts + 2, fw, o, cnt, c->feat, c->y,
-                         c->featc, c->yc);
-      CL_HIP(hipGetLastError());
+      hipLaunchKernelGGL(add_u32, dim3(1), dim3(64), 0, s, c->counts + kSurvTotal,
+                         c->counts + kSurv);
       for (int k = 2; k < M && rc == CE_OK; ++k)
-        rc = ce::mlp_forward_device(c->members[k], c->featc, static_cast<int64_t>(n_surv),
-                                    c->yc + static_cast<int64_t>(k) * n_surv * o, s);
+        rc = ce::mlp_forward_device(c->members[k], c->feat, static_cast<int64_t>(cnt),
+                                    c->yc + static_cast<int64_t>(k - 2) * cnt * o, s,
+                                    c->counts + kSurv, c->rows);
       if (rc != CE_OK) break;
-      g.y = c->yc;
-      g.count = n_surv;
+      g.y2 = c->yc;
+      g.split = 2;
+      g.count_dev = c->counts + kSurv;
       g.rows = c->rows;
       g.count_cands = 0;  // counted by the pre-gate
-    } else {
-      st.member_rows += static_cast<uint64_t>(M) * cnt;
     }
     rc = launch_gate(c, g, s);
   }
   if (rc != CE_OK) return rc;
-  uint32_t counts[2] = {0, 0};
-  CL_HIP(hipMemcpyAsync(counts, c->counts, sizeof(counts), hipMemcpyDeviceToHost, s));
+  std::vector<uint32_t> counts(kCounters, 0u);
+  CL_HIP(hipMemcpyAsync(counts.data(), c->counts, kCounters * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, s));
   CL_HIP(hipEventRecord(c->ev[2], s));
   CL_HIP(hipStreamSynchronize(s));
-  const uint32_t n_fb = counts[0];
+  const uint32_t n_fb = counts[kFallback];
   st.fallback = n_fb;
-  st.candidates = counts[1];
+  st.candidates = counts[kCands];
+  if (pregate) st.member_rows += static_cast<uint64_t>(M - 2) * counts[kSurvTotal];
   // 3. fallback: continue the queued sims from their saved state
   if (n_fb > 0) {
     ce_run_config full = *cfg;

@@ -342,6 +342,8 @@ constexpr float kLog2e = 1.4426950408889634f;
 struct FusedArgs {
   const float* x;     // [rows][w0] fp32
   int64_t rows;
+  const uint32_t* rows_dev;  // or nullptr: the row count, at most `rows`, read on the device
+  const uint32_t* idx;       // or nullptr: sample m is row idx[m] of x (else row m)
   int w0;             // inputs; columns w0 and w0 + 1 of layer 0 hold its bias
   int w_out;          // outputs (<= 32)
   const uint16_t* W;  // the weight stream
@@ -459,13 +461,17 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
   __shared__ __align__(16) uint16_t lds[2][F::LDS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, ln = lane & 31, h = lane >> 5;
   const int64_t m = static_cast<int64_t>(blockIdx.x) * 128 + wave * 32 + ln;  // this lane's sample
+  int64_t rows = a.rows;
+  if (a.rows_dev != nullptr) rows = min(rows, static_cast<int64_t>(*a.rows_dev));
+  if (static_cast<int64_t>(blockIdx.x) * 128 >= rows) return;  // the whole workgroup
 
   u32x4 rg[F::PD][F::CH];
   fz_fetch<F, 0>(a.W, rg[0]);
   // inputs, then the two constant-1 columns of layer 0's bias (branch free: every
   // load is in bounds, rows past the end read the last row and are zeroed)
   bf16x8 xin[K0S];
-  const float* xrow = a.x + (m < a.rows ? m : a.rows - 1) * a.w0;
+  const int64_t mx = m < rows ? m : rows - 1;
+  const float* xrow = a.x + (a.idx != nullptr ? static_cast<int64_t>(a.idx[mx]) : mx) * a.w0;
 #pragma unroll
   for (int ks = 0; ks < K0S; ++ks) {
     uint32_t d[4];
@@ -477,7 +483,7 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
         const int k = 16 * ks + 8 * h + 2 * j + u;
         const float xv = xrow[k < a.w0 ? k : a.w0 - 1];
         const float fv = k < a.w0 ? xv : (k < a.w0 + 2 ? 1.0f : 0.0f);
-        f[u] = m < a.rows ? fv : 0.0f;
+        f[u] = m < rows ? fv : 0.0f;
       }
       d[j] = pack_bf16(f[0], f[1]);
     }
@@ -542,7 +548,7 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
     if constexpr (F::PD == 2) bcur = bnext;
   });
   // prev holds the output block: log2(e) (W_out a + b_out) with a = log2(e) SiLU
-  if (m < a.rows) {
+  if (m < rows) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n = 8 * (r >> 2) + 4 * h + (r & 3);
@@ -1128,8 +1134,14 @@ int mlp_width(const ce_mlp* m, int layer) {
 }
 int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 
-int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s) {
+bool mlp_fused_ready(const ce_mlp* m) { return m != nullptr && m->fused_h && m->use_fused; }
+
+int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s,
+                       const uint32_t* rows_dev, const uint32_t* idx) {
   ce::clear_stale_error();
+  if ((rows_dev != nullptr || idx != nullptr) && !(m->fused_h && m->use_fused))
+    return fail(CE_ERR_INVALID_ARGUMENT,
+                "a device row count or row index needs the fused inference kernel");
   if (m->fused_h && m->use_fused && B > 0) {
     const int H = m->fused_h, L = m->L, k0 = round_up(m->widths[0] + 2, 64);
     if (m->wp_stale) {
@@ -1148,6 +1160,8 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     FusedArgs a{};
     a.x = x;
     a.rows = B;
+    a.rows_dev = rows_dev;
+    a.idx = idx;
     a.w0 = m->widths[0];
     a.w_out = m->widths[L];
     a.W = m->Wp_all;

@@ -162,14 +162,18 @@ def test_features_and_prior_vs_oracle(gpu):
         np.testing.assert_allclose(f[s], closure_ref.features(x, v, m, q), rtol=2e-7, atol=1e-6)
 
 
-@pytest.fixture(scope="module")
-def trained(gpu):
-    """A small ensemble trained on engine data (the real one is in bench.py)."""
+@pytest.fixture(scope="module", params=[(128, 128), (256, 256)], ids=["layers", "fused"])
+def trained(gpu, request):
+    """A small ensemble trained on engine data (the real one is in bench.py).  At
+    width 256 inference runs through the fused kernel, and the survivors of the
+    early rejection keep their count on the device."""
     ce = gpu
     import coulomb_explosion_amd.closure_train as ct
     cfg = ce.RunConfig(**RULE)
     data = ct.make_data(ce, ch4(ce), 100000, 4242, cfg, T_CUT, 2.5, 1.0, 1e-6, 1e-9)
-    ens = ct.fit(ce, data, members=3, hidden=(128, 128), epochs=20, batch=1024, lr=2e-3)
+    ens = ct.fit(ce, data, members=3, hidden=request.param, epochs=20, batch=1024, lr=2e-3)
+    fused = [m.set_fused(True) for m in ens.members]
+    assert all(fused) == (request.param[0] == 256) and len(set(fused)) == 1
     return ct, cfg, ens
 
 
@@ -217,17 +221,25 @@ def test_closure_gate_follows_spread(gpu, trained):
     eng = engine(ce, 3000, 123)
     eng.run(replace(cfg, t_max=T_CUT))
     feat, prior = clo.features(eng)
-    _, rel = ct.predict(ens, feat, prior)
+    mean, rel = ct.predict(ens, feat, prior)
     tol = float(np.quantile(rel, 0.5))
     res = clo.run(eng, cfg, T_CUT, tol)
     acc = res.status == ce.SIM_CLOSURE
     far = np.abs(rel - tol) > 1e-3 * tol  # away from the threshold (rounding)
     np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
+    # the accepted sims take the mean of all members (the third one on compact rows)
+    np.testing.assert_allclose(res.momenta.reshape(-1, 5, 3)[acc], mean[acc], rtol=1e-6,
+                               atol=1e-6 * np.abs(mean).max())
+    assert np.all(res.status[~acc] == ce.SIM_OK_TMAX) and np.all(res.t_final[~acc] == 1e4)
     st = clo.stats()
     assert st["fallback"] == np.count_nonzero(~acc)
+    m, cand = len(ens.members), st["candidates"]
+    if not ens.members[0].set_fused(True):
+        # layer-by-layer inference: every member runs on every sim
+        assert st["member_rows"] == m * cand
+        return
     # the first two members decided part of the sims (early rejection), the third
     # member ran on the rest, and the decisions above are still the full gate's
-    m, cand = len(ens.members), st["candidates"]
     assert 2 * cand < st["member_rows"] < m * cand
     # at the closed gate the first two members decide every sim that they disagree on;
     # wide open, every member runs on every sim
@@ -235,7 +247,35 @@ def test_closure_gate_follows_spread(gpu, trained):
     assert clo.stats()["member_rows"] == 2 * cand
     clo.run(eng, cfg, T_CUT, 1e30)
     assert clo.stats()["member_rows"] == m * cand
-    assert np.all(res.status[~acc] == ce.SIM_OK_TMAX) and np.all(res.t_final[~acc] == 1e4)
+
+
+def test_closure_gate_across_chunks(gpu, trained):
+    """Over 2.2M sims the ensemble runs in three chunks of at most 1M: every
+    chunk's survivors, compact rows and counts land on the right sims."""
+    ce = gpu
+    ct, cfg, ens = trained
+    n = 2_200_000
+    clo = ct.attach(ce, ch4(ce), ens)
+    eng = engine(ce, n, 321)
+    cand = eng.run(replace(cfg, t_max=T_CUT)).status == ce.SIM_OK_TMAX
+    feat, prior = clo.features(eng)
+    mean, rel = ct.predict(ens, feat, prior)
+    del feat
+    tol = float(np.quantile(rel, 0.3))
+    res = clo.run(eng, cfg, T_CUT, tol)
+    acc = res.status == ce.SIM_CLOSURE
+    far = cand & (np.abs(rel - tol) > 1e-3 * tol)
+    np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
+    assert not np.any(acc[~cand])
+    np.testing.assert_allclose(res.momenta.reshape(-1, 5, 3)[acc], mean[acc], rtol=1e-6,
+                               atol=1e-6 * np.abs(mean).max())
+    st = clo.stats()
+    m, nc = len(ens.members), np.count_nonzero(cand)
+    assert st["candidates"] == nc and st["fallback"] == nc - np.count_nonzero(acc)
+    if ens.members[0].set_fused(True):
+        assert 2 * nc < st["member_rows"] < m * nc
+    else:
+        assert st["member_rows"] == m * nc
 
 
 def test_closure_per_sim_charges_n10(gpu):
