@@ -338,6 +338,9 @@ __global__ void adam(float* p, const float* g, float* m, float* v, int64_t n, fl
 constexpr int kFusedMaxLayers = 6;     // H = 256
 constexpr int kFusedMaxLayers512 = 5;  // H = 512 (code size)
 constexpr float kLog2e = 1.4426950408889634f;
+// sched_barrier mask: VALU, SALU and transcendental instructions may cross it
+// (MFMA, LDS and global memory instructions may not).
+constexpr int kSchedAlu = 0x2 | 0x4 | 0x400;
 
 struct FusedArgs {
   const float* x;     // [rows][w0] fp32
@@ -379,6 +382,10 @@ struct Fz {
   // Tiles in flight in registers: two ahead at H = 256; one at H = 512, whose
   // activations (two 128-register sets) leave no room for a second.
   static constexpr int PD = H == 256 ? 2 : 1;
+  // Weight fragments read ahead of the MFMAs: at H = 512 (one wave per SIMD, no
+  // other wave to cover the LDS latency) three; at H = 256 the compiler's one
+  // (two waves per SIMD; three measured no faster).
+  static constexpr int RA = H == 256 ? 1 : 3;
   static_assert(K0 % 32 == 0 && B0 * K0S == KS && T0 * B0 == NB, "tile geometry");
   static constexpr int tile_of(int g) { return g < NB ? g / B0 : T0 + (g - NB); }
   static constexpr int pos_of(int g) { return g < NB ? g % B0 : 0; }
@@ -532,9 +539,18 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
         acc = bcur;
       }
       const uint16_t* arow = T + (32 * F::pos_of(g) + ln) * stride + 8 * h;
+      // Weight fragments RA - 1 steps ahead of their MFMA (a ring in registers); the
+      // scheduling barrier keeps the LDS reads there (ALU work may still cross).
+      constexpr int RA = ksteps < F::RA ? ksteps : F::RA;
+      bf16x8 wr[RA];
+#pragma unroll
+      for (int p = 0; p < RA - 1; ++p) wr[p] = *reinterpret_cast<const bf16x8*>(arow + 16 * p);
 #pragma unroll
       for (int ks = 0; ks < ksteps; ++ks) {
-        const bf16x8 w = *reinterpret_cast<const bf16x8*>(arow + 16 * ks);
+        if (ks + RA - 1 < ksteps)
+          wr[(ks + RA - 1) % RA] = *reinterpret_cast<const bf16x8*>(arow + 16 * (ks + RA - 1));
+        if constexpr (RA > 1) __builtin_amdgcn_sched_barrier(kSchedAlu);
+        const bf16x8 w = wr[ks % RA];
         if constexpr (layer == 0)
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, xin[ks], acc, 0, 0, 0);
         else
