@@ -53,23 +53,31 @@ struct FeatureArgs {
   const float* inv_scale;
   uint64_t first, count;  // sims [first, first + count)
   int all;                // 1: every sim, 0: only sims stopped by the cut
-  float* feat;            // [count][8n]
-  double* prior;          // [count][3n]
+  float* feat;            // [count][8n], or column-major [8n][ld]
+  double* prior;          // [count][3n], or column-major [3n][ld]
+  uint64_t ld;            // 0: row-major; else the column-major leading dimension
 };
 
 constexpr int kFeatBlock = 128;  // sims per block of the feature kernel
 
-// One thread per sim; the rows are staged in LDS and written back coalesced.
-template <int N>
+// One thread per sim.  Column-major (COL, the closure run): every thread writes
+// its sim's column entries, coalesced across the block.  Row-major (the host
+// feature call): the rows are staged in LDS and written back coalesced.  The
+// state stays in single precision in registers (it is stored so) and each use
+// widens it.  (Staging the block's saved positions and velocities through LDS
+// for coalesced loads measured slower: 112 -> 178 us per 1M CH4 sims.)
+template <int N, bool COL>
 __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs a,
                                                                const ClosureConst cc) {
-  __shared__ float fs[kFeatBlock][8 * N];
-  __shared__ double ps[kFeatBlock][3 * N];
   const uint64_t r0 = blockIdx.x * uint64_t(kFeatBlock);
   const uint64_t r = r0 + threadIdx.x;
   const uint64_t g = a.first + r;
-  if (r < a.count && (a.all || a.status[g] == ce::kSimOkTmax)) {
-    double x[3 * N], v[3 * N], q[N], p[3 * N];
+  float fs_row[COL ? 1 : 8 * N];   // unused when COL
+  double ps_row[COL ? 1 : 3 * N];  // unused when COL
+  const bool active = r < a.count && (a.all || a.status[g] == ce::kSimOkTmax);
+  if (active) {
+    float x[3 * N], v[3 * N];
+    double q[N];
     double cm[3] = {0.0, 0.0, 0.0}, mt = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -79,14 +87,15 @@ __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs
       for (int c = 0; c < 3; ++c) {
         x[3 * i + c] = a.x[g * 3 * N + 3 * i + c];
         v[3 * i + c] = a.v[g * 3 * N + 3 * i + c];
-        p[3 * i + c] = cc.m[i] * v[3 * i + c];
-        cm[c] += cc.m[i] * x[3 * i + c];
+        cm[c] += cc.m[i] * static_cast<double>(x[3 * i + c]);
       }
     }
     // prior: straight-line impulse of every pair, then KE rescaled to E0
     double pp[3 * N];
 #pragma unroll
-    for (int j = 0; j < 3 * N; ++j) pp[j] = p[j];
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) pp[3 * i + c] = cc.m[i] * static_cast<double>(v[3 * i + c]);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
 #pragma unroll
@@ -95,8 +104,8 @@ __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs
         double c2 = 0.0, a2 = 0.0, b = 0.0;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-          rr[c] = x[3 * i + c] - x[3 * j + c];
-          uu[c] = v[3 * i + c] - v[3 * j + c];
+          rr[c] = static_cast<double>(x[3 * i + c]) - static_cast<double>(x[3 * j + c]);
+          uu[c] = static_cast<double>(v[3 * i + c]) - static_cast<double>(v[3 * j + c]);
           c2 += rr[c] * rr[c];
           a2 += uu[c] * uu[c];
           b += rr[c] * uu[c];
@@ -104,8 +113,9 @@ __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs
         const double rn = sqrt(c2), un = sqrt(a2);
         const double w = rn * un + b;
         if (!(rn > 0.0) || !(un > 0.0) || !(w > 1e-300)) continue;
-        const double s = cc.k * q[i] * q[j] / w;
-        const double sr = s / rn, su = s / un;
+        // s / rn and s / un with s = k q_i q_j / w, through one division
+        const double kq = cc.k * q[i] * q[j] / (w * rn * un);
+        const double sr = kq * un, su = kq * rn;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double J = sr * rr[c] + su * uu[c];
@@ -121,31 +131,53 @@ __global__ void __launch_bounds__(kFeatBlock) closure_features(const FeatureArgs
                    pp[3 * i + 2] * pp[3 * i + 2]) / cc.m[i];
     const double sc = (ke > 0.0 && a.e0[g] > 0.0) ? sqrt(a.e0[g] / ke) : 1.0;
 #pragma unroll
-    for (int j = 0; j < 3 * N; ++j) ps[threadIdx.x][j] = pp[j] * sc;
-    // features
-    float f[8 * N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        f[6 * i + c] = static_cast<float>(x[3 * i + c] - cm[c] / mt);
-        f[6 * i + 3 + c] = static_cast<float>(p[3 * i + c]);
-      }
-      f[6 * N + i] = static_cast<float>(q[i]);
-      f[7 * N + i] = static_cast<float>(cc.m[i]);
+    for (int j = 0; j < 3 * N; ++j) {
+      if constexpr (COL)
+        a.prior[j * a.ld + r] = pp[j] * sc;
+      else
+        ps_row[j] = pp[j] * sc;
     }
+    // features
 #pragma unroll
-    for (int j = 0; j < 8 * N; ++j)
-      fs[threadIdx.x][j] = a.shift != nullptr ? (f[j] - a.shift[j]) * a.inv_scale[j] : f[j];
+    for (int j = 0; j < 8 * N; ++j) {
+      const int i = j < 6 * N ? j / 6 : (j < 7 * N ? j - 6 * N : j - 7 * N);
+      const int c = j < 6 * N ? j % 6 : 0;
+      float f;
+      if (j >= 7 * N)
+        f = static_cast<float>(cc.m[i]);
+      else if (j >= 6 * N)
+        f = static_cast<float>(q[i]);
+      else if (c < 3)
+        f = static_cast<float>(static_cast<double>(x[3 * i + c]) - cm[c] / mt);
+      else
+        f = static_cast<float>(cc.m[i] * static_cast<double>(v[3 * i + c - 3]));
+      if (a.shift != nullptr) f = (f - a.shift[j]) * a.inv_scale[j];
+      if constexpr (COL)
+        a.feat[j * a.ld + r] = f;
+      else
+        fs_row[j] = f;
+    }
   }
-  __syncthreads();
-  // the block's rows are contiguous in feat / prior (rows not computed carry
-  // whatever LDS held: nothing reads them)
-  const uint64_t rows = a.count - r0 < kFeatBlock ? a.count - r0 : kFeatBlock;
-  const float* fsf = &fs[0][0];
-  for (uint64_t e = threadIdx.x; e < rows * 8 * N; e += kFeatBlock) a.feat[r0 * 8 * N + e] = fsf[e];
-  const double* psf = &ps[0][0];
-  for (uint64_t e = threadIdx.x; e < rows * 3 * N; e += kFeatBlock) a.prior[r0 * 3 * N + e] = psf[e];
+  if constexpr (!COL) {
+    __shared__ float fs[kFeatBlock][8 * N];
+    __shared__ double ps[kFeatBlock][3 * N];
+    if (active) {
+#pragma unroll
+      for (int j = 0; j < 8 * N; ++j) fs[threadIdx.x][j] = fs_row[j];
+#pragma unroll
+      for (int j = 0; j < 3 * N; ++j) ps[threadIdx.x][j] = ps_row[j];
+    }
+    __syncthreads();
+    // the block's rows are contiguous in feat / prior (rows not computed carry
+    // whatever LDS held: nothing reads them)
+    const uint64_t rows = a.count - r0 < kFeatBlock ? a.count - r0 : kFeatBlock;
+    const float* fsf = &fs[0][0];
+    for (uint64_t e = threadIdx.x; e < rows * 8 * N; e += kFeatBlock)
+      a.feat[r0 * 8 * N + e] = fsf[e];
+    const double* psf = &ps[0][0];
+    for (uint64_t e = threadIdx.x; e < rows * 3 * N; e += kFeatBlock)
+      a.prior[r0 * 3 * N + e] = psf[e];
+  }
 }
 
 // The counters, 256 bytes apart (atomics on one line serialise with each other):
@@ -159,10 +191,13 @@ constexpr int kGateBlock = 1024;
 template <int N>
 constexpr int gate_block() { return N <= 6 ? kGateBlock : kGateBlock / 2; }
 
+// Layout of the members' outputs and the prior: entry j of row R at j * es + R * rs
+// (row-major: es = 1, rs = 3n; column-major: es = the chunk size, rs = 1), member
+// k's outputs ms entries after member k - 1's.
 struct GateArgs {
-  const double* prior;      // [count][3n]
-  const float* y;           // members [0, split): [split][stride][3n] at chunk rows
-  const float* y2;          // members [split, members): [..][stride][3n] at compact rows
+  const double* prior;      // [chunk rows]
+  const float* y;           // members [0, split) at chunk rows
+  const float* y2;          // members [split, members) at compact rows
   const float* out_scale;   // [3n]
   int members;
   uint64_t first, count;
@@ -176,7 +211,7 @@ struct GateArgs {
   // row rows[r] (prior, status, members [0, split)); nullptr = the chunk itself.
   const uint32_t* rows;
   const uint32_t* count_dev;  // or nullptr: the rows are min(count, *count_dev), read here
-  uint64_t stride;          // rows between members in y and y2
+  uint64_t es, rs, ms;
   int32_t split;
   int32_t count_cands;      // add the candidates to kCands (not when the pre-gate has)
 };
@@ -224,18 +259,17 @@ __global__ void __launch_bounds__(gate_block<N>()) closure_gate(const GateArgs a
   bool ok = false;
   if (cand) {
     // member-major passes: the 3N loads of a member are independent of each other
-    const uint64_t cs = a.stride * 3 * N;  // member stride
-    const float* yr = a.y + rr * 3 * N;
-    const float* yr2 = a.y2 + r * 3 * N - a.split * cs;  // member k >= split at yr2 + k cs
+    const float* yr = a.y + rr * a.rs;
+    const float* yr2 = a.y2 + r * a.rs - a.split * a.ms;  // member k >= split at yr2 + k ms
     const double inv_m = a.members > 0 ? 1.0 / a.members : 0.0;
     const double inv_m1 = a.members > 1 ? 1.0 / (a.members - 1) : 1.0;
     double pm[3 * N];
 #pragma unroll
     for (int j = 0; j < 3 * N; ++j) pm[j] = 0.0;
     for (int k = 0; k < a.members; ++k) {
-      const float* yk = (k < a.split ? yr : yr2) + k * cs;
+      const float* yk = (k < a.split ? yr : yr2) + k * a.ms;
 #pragma unroll
-      for (int j = 0; j < 3 * N; ++j) pm[j] += yk[j];
+      for (int j = 0; j < 3 * N; ++j) pm[j] += yk[j * a.es];
     }
 #pragma unroll
     for (int j = 0; j < 3 * N; ++j) pm[j] *= inv_m;
@@ -243,10 +277,10 @@ __global__ void __launch_bounds__(gate_block<N>()) closure_gate(const GateArgs a
 #pragma unroll
     for (int i = 0; i < N; ++i) var[i] = 0.0;
     for (int k = 0; k < a.members; ++k) {
-      const float* yk = (k < a.split ? yr : yr2) + k * cs;
+      const float* yk = (k < a.split ? yr : yr2) + k * a.ms;
 #pragma unroll
       for (int j = 0; j < 3 * N; ++j) {
-        const double d = (yk[j] - pm[j]) * a.out_scale[j];
+        const double d = (yk[j * a.es] - pm[j]) * a.out_scale[j];
         var[j / 3] += d * d;
       }
     }
@@ -257,7 +291,7 @@ __global__ void __launch_bounds__(gate_block<N>()) closure_gate(const GateArgs a
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const int j = 3 * i + c;
-        pm[j] = a.prior[rr * 3 * N + j] + pm[j] * a.out_scale[j];  // predicted momentum
+        pm[j] = a.prior[j * a.es + rr * a.rs] + pm[j] * a.out_scale[j];  // predicted momentum
         p2 += pm[j] * pm[j];
       }
       ok = ok && (var[i] * inv_m1 <= a.tol2 * p2);  // false for NaN
@@ -289,8 +323,9 @@ __global__ void __launch_bounds__(gate_block<N>()) closure_gate(const GateArgs a
 // and the full gate.  A relative margin of 1e-9 keeps rounding (FP64 here) on
 // the side of the full gate.
 struct PregateArgs {
-  const double* prior;      // [count][3n]
-  const float* y;           // members 0 and 1: [2][count][3n] (member stride count 3n)
+  const double* prior;      // entry j of row R at j es + R rs (as GateArgs)
+  const float* y;           // members 0 and 1, ms apart
+  uint64_t es, rs, ms;
   const float* out_scale;   // [3n]
   uint64_t first, count;
   double c;                 // (members - 1) tol^2, < 1
@@ -303,14 +338,13 @@ struct PregateArgs {
 
 template <int N>
 __global__ void __launch_bounds__(kGateBlock) closure_pregate(const PregateArgs a) {
-  constexpr int O = 3 * N;
   const uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
   const uint64_t g = a.first + r;
   const bool cand = r < a.count && a.status[g] == ce::kSimOkTmax;
   bool reject = false;
   if (cand) {
-    const float* y0 = a.y + r * O;
-    const float* y1 = y0 + a.count * O;
+    const float* y0 = a.y + r * a.rs;
+    const float* y1 = y0 + a.ms;
     const double bound = a.c * a.k / (a.k - a.c) * (1.0 + 1e-9);
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -318,10 +352,10 @@ __global__ void __launch_bounds__(kGateBlock) closure_pregate(const PregateArgs 
 #pragma unroll
       for (int cc = 0; cc < 3; ++cc) {
         const int j = 3 * i + cc;
-        const double z0 = static_cast<double>(y0[j]) * a.out_scale[j];
-        const double z1 = static_cast<double>(y1[j]) * a.out_scale[j];
+        const double z0 = static_cast<double>(y0[j * a.es]) * a.out_scale[j];
+        const double z1 = static_cast<double>(y1[j * a.es]) * a.out_scale[j];
         const double d = z0 - z1;
-        const double m = a.prior[r * O + j] + 0.5 * (z0 + z1);
+        const double m = a.prior[j * a.es + r * a.rs] + 0.5 * (z0 + z1);
         d12 += 0.5 * d * d;
         a2 += m * m;
       }
@@ -452,13 +486,13 @@ unsigned gate_blocks(uint64_t n, int b) { return static_cast<unsigned>((n + b - 
 int launch_features(ce_closure* c, const FeatureArgs& a, hipStream_t s) {
   ce::clear_stale_error();
   switch (c->n) {
-#define CL_F(NN)                                                                      \
-  case NN:                                                                            \
-    hipLaunchKernelGGL(closure_features<NN>,                                          \
-                       dim3(static_cast<unsigned>((a.count + kFeatBlock - 1) / kFeatBlock)),  \
-                       dim3(kFeatBlock), 0, s, a,                                         \
-                       c->cc);                                                        \
-    break;
+#define CL_F(NN)                                                                         \
+  case NN: {                                                                             \
+    auto kern = a.ld > 0 ? closure_features<NN, true> : closure_features<NN, false>;     \
+    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>((a.count + kFeatBlock - 1) / kFeatBlock)), \
+                       dim3(kFeatBlock), 0, s, a, c->cc);                                \
+    break;                                                                               \
+  }
     CL_F(1) CL_F(2) CL_F(3) CL_F(4) CL_F(5) CL_F(6) CL_F(7) CL_F(8) CL_F(9) CL_F(10)
 #undef CL_F
     default: return fail(CE_ERR_INVALID_ARGUMENT, "n_atoms out of range");
@@ -506,7 +540,7 @@ int check_engine(const ce_closure* c, const ce_engine* e) {
 }
 
 FeatureArgs feature_args(const ce_closure* c, const ce_engine* e, uint64_t first, uint64_t count,
-                         int all, bool normalize) {
+                         int all, bool normalize, uint64_t ld) {
   FeatureArgs a{};
   a.x = static_cast<const float*>(e->state_x);
   a.v = static_cast<const float*>(e->state_v);
@@ -520,6 +554,7 @@ FeatureArgs feature_args(const ce_closure* c, const ce_engine* e, uint64_t first
   a.all = all;
   a.feat = c->feat;
   a.prior = c->prior;
+  a.ld = ld;
   return a;
 }
 
@@ -613,7 +648,7 @@ int ce_closure_features(ce_closure* c, ce_engine* e, float* features, double* pr
   if (rc != CE_OK) return rc;
   for (uint64_t f0 = 0; f0 < n_sims; f0 += chunk) {
     const uint64_t cnt = std::min<uint64_t>(chunk, n_sims - f0);
-    rc = launch_features(c, feature_args(c, e, f0, cnt, 1, false), e->stream);
+    rc = launch_features(c, feature_args(c, e, f0, cnt, 1, false, 0), e->stream);
     if (rc != CE_OK) return rc;
     if (features)
       CL_HIP(hipMemcpyAsync(features + f0 * 8 * c->n, c->feat, cnt * 8 * c->n * sizeof(float),
@@ -674,16 +709,22 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
   // the others run on the survivors, whose count stays on the device: the member
   // passes read their rows through the survivor list, the gate reads the first
   // two members at the chunk rows and the others at the compact rows.
+  // With the fused kernel the features, the prior and the members' outputs are
+  // column-major (chunk rows contiguous): every kernel of the phase reads and
+  // writes them coalesced.
   const double c_bound = (M - 1) * gate_tol * gate_tol;
-  bool pregate = M >= 3 && c_bound < 1.0;
-  for (int k = 2; k < M; ++k) pregate = pregate && ce::mlp_fused_ready(c->members[k]);
+  bool fused = true;
+  for (int k = 0; k < M; ++k) fused = fused && ce::mlp_fused_ready(c->members[k]);
+  const bool pregate = fused && M >= 3 && c_bound < 1.0;
   for (uint64_t f0 = 0; f0 < n && rc == CE_OK; f0 += chunk) {
     const uint64_t cnt = std::min<uint64_t>(chunk, n - f0);
-    rc = launch_features(c, feature_args(c, e, f0, cnt, 0, true), s);
+    rc = launch_features(c, feature_args(c, e, f0, cnt, 0, true, fused ? cnt : 0), s);
+    ce::FusedIo io{};
+    if (fused) io.ldx = io.ldy = static_cast<int64_t>(cnt);
     const int first_pass = pregate ? 2 : M;
     for (int k = 0; k < first_pass && rc == CE_OK; ++k)
       rc = ce::mlp_forward_device(c->members[k], c->feat, static_cast<int64_t>(cnt),
-                                  c->y + static_cast<int64_t>(k) * cnt * o, s);
+                                  c->y + static_cast<int64_t>(k) * cnt * o, s, io);
     if (rc != CE_OK) break;
     GateArgs g{};
     g.prior = c->prior;
@@ -692,7 +733,9 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
     g.members = M;
     g.first = f0;
     g.count = cnt;
-    g.stride = cnt;
+    g.es = fused ? cnt : 1;
+    g.rs = fused ? 1 : static_cast<uint64_t>(o);
+    g.ms = cnt * o;
     g.split = M;
     g.tol2 = gate_tol * gate_tol;
     g.status = e->status;
@@ -708,6 +751,9 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
       PregateArgs pa{};
       pa.prior = c->prior;
       pa.y = c->y;
+      pa.es = g.es;
+      pa.rs = g.rs;
+      pa.ms = g.ms;
       pa.out_scale = c->out_scale;
       pa.first = f0;
       pa.count = cnt;
@@ -721,10 +767,11 @@ int ce_closure_run(ce_closure* c, ce_engine* e, const ce_run_config* cfg, double
       if (rc != CE_OK) break;
       hipLaunchKernelGGL(add_u32, dim3(1), dim3(64), 0, s, c->counts + kSurvTotal,
                          c->counts + kSurv);
+      io.rows_dev = c->counts + kSurv;
+      io.idx = c->rows;
       for (int k = 2; k < M && rc == CE_OK; ++k)
         rc = ce::mlp_forward_device(c->members[k], c->feat, static_cast<int64_t>(cnt),
-                                    c->yc + static_cast<int64_t>(k - 2) * cnt * o, s,
-                                    c->counts + kSurv, c->rows);
+                                    c->yc + static_cast<int64_t>(k - 2) * cnt * o, s, io);
       if (rc != CE_OK) break;
       g.y2 = c->yc;
       g.split = 2;

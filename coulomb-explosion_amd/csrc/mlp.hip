@@ -347,6 +347,7 @@ struct FusedArgs {
   int64_t rows;
   const uint32_t* rows_dev;  // or nullptr: the row count, at most `rows`, read on the device
   const uint32_t* idx;       // or nullptr: sample m is row idx[m] of x (else row m)
+  int64_t ldx, ldy;          // > 0: x / y column-major with these leading dimensions
   int w0;             // inputs; columns w0 and w0 + 1 of layer 0 hold its bias
   int w_out;          // outputs (<= 32)
   const uint16_t* W;  // the weight stream
@@ -478,7 +479,10 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
   // load is in bounds, rows past the end read the last row and are zeroed)
   bf16x8 xin[K0S];
   const int64_t mx = m < rows ? m : rows - 1;
-  const float* xrow = a.x + (a.idx != nullptr ? static_cast<int64_t>(a.idx[mx]) : mx) * a.w0;
+  const int64_t xr = a.idx != nullptr ? static_cast<int64_t>(a.idx[mx]) : mx;
+  // input k of this sample at xs + k * xk (row- or column-major)
+  const float* xs = a.x + (a.ldx > 0 ? xr : xr * a.w0);
+  const int64_t xk = a.ldx > 0 ? a.ldx : 1;
 #pragma unroll
   for (int ks = 0; ks < K0S; ++ks) {
     uint32_t d[4];
@@ -488,7 +492,7 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
         const int k = 16 * ks + 8 * h + 2 * j + u;
-        const float xv = xrow[k < a.w0 ? k : a.w0 - 1];
+        const float xv = xs[(k < a.w0 ? k : a.w0 - 1) * xk];
         const float fv = k < a.w0 ? xv : (k < a.w0 + 2 ? 1.0f : 0.0f);
         f[u] = m < rows ? fv : 0.0f;
       }
@@ -568,7 +572,7 @@ __global__ void __launch_bounds__(256, H == 256 ? 2 : 1) mlp_fused_forward(const
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int n = 8 * (r >> 2) + 4 * h + (r & 3);
-      if (n < a.w_out) a.y[m * a.w_out + n] = prev[r] * (1.0f / kLog2e);
+      if (n < a.w_out) a.y[a.ldy > 0 ? n * a.ldy + m : m * a.w_out + n] = prev[r] * (1.0f / kLog2e);
     }
   }
 }
@@ -1153,11 +1157,13 @@ int mlp_layers(const ce_mlp* m) { return m != nullptr ? m->L : 0; }
 bool mlp_fused_ready(const ce_mlp* m) { return m != nullptr && m->fused_h && m->use_fused; }
 
 int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream_t s,
-                       const uint32_t* rows_dev, const uint32_t* idx) {
+                       const FusedIo& io) {
   ce::clear_stale_error();
-  if ((rows_dev != nullptr || idx != nullptr) && !(m->fused_h && m->use_fused))
+  const bool plain_io = io.rows_dev == nullptr && io.idx == nullptr && io.ldx == 0 && io.ldy == 0;
+  if (!plain_io && !(m->fused_h && m->use_fused))
     return fail(CE_ERR_INVALID_ARGUMENT,
-                "a device row count or row index needs the fused inference kernel");
+                "a device row count, a row index or a column-major layout needs the fused "
+                "inference kernel");
   if (m->fused_h && m->use_fused && B > 0) {
     const int H = m->fused_h, L = m->L, k0 = round_up(m->widths[0] + 2, 64);
     if (m->wp_stale) {
@@ -1176,8 +1182,10 @@ int mlp_forward_device(ce_mlp* m, const float* x, int64_t B, float* y, hipStream
     FusedArgs a{};
     a.x = x;
     a.rows = B;
-    a.rows_dev = rows_dev;
-    a.idx = idx;
+    a.rows_dev = io.rows_dev;
+    a.idx = io.idx;
+    a.ldx = io.ldx;
+    a.ldy = io.ldy;
     a.w0 = m->widths[0];
     a.w_out = m->widths[L];
     a.W = m->Wp_all;
