@@ -314,3 +314,40 @@ def test_closure_per_sim_charges_n10(gpu):
     np.testing.assert_array_equal(res0["momenta"], resumed["momenta"])
     res1 = clo.run(engine(), cfg, T_CUT, 1e30)[0]
     assert np.all(res1["status"] == ce.SIM_CLOSURE) and clo.stats()["fallback"] == 0
+
+
+def test_closure_per_sim_charges_n10_fused(gpu):
+    """N=10 with per-sim charges and members of the fused 512-wide shape: the
+    early rejection, the column-major phase arrays and the survivors' compact rows
+    give the full gate's decisions, and the accepted sims take the members' mean."""
+    ce = gpu
+    import coulomb_explosion_amd.closure_train as ct
+    C, O, H = 21874.66, 29156.95, 1837.15
+    mol = ce.Molecule([ce.Atom("X", m, 1.0) for m in [C, C, O, O] + [H] * 6])
+    cfg = ce.RunConfig(**RULE)
+    eng = ce.MixedBatch([ce.Species(mol, 3000, 3.5, 0.9, charge_levels=3)], 1.0, 1e-6, 1e-9)
+    eng.sample(2025)
+    eng.enable_state()
+    cand = eng.run(replace(cfg, t_max=T_CUT))[0]["status"] == ce.SIM_OK_TMAX
+    clo = ce.Closure(mol)
+    f, pr = clo.features(eng)
+    nets = [ce.MLP([80, 512, 512, 30], seed=11 + k) for k in range(3)]
+    assert all(n.set_fused(True) for n in nets)
+    ens = ct.Ensemble(nets, f.mean(0).astype(np.float32),
+                      np.maximum(f.std(0), 1e-6).astype(np.float32),
+                      np.full(30, 0.5, np.float32), [], 0.0)
+    clo = ct.attach(ce, mol, ens)
+    mean, rel = ct.predict(ens, f, pr)
+    tol = float(np.quantile(rel, 0.5))
+    res = clo.run(eng, cfg, T_CUT, tol)[0]
+    acc = res["status"] == ce.SIM_CLOSURE
+    far = cand & (np.abs(rel - tol) > 1e-3 * tol)
+    np.testing.assert_array_equal(acc[far], (rel <= tol)[far])
+    assert not np.any(acc[~cand])
+    np.testing.assert_allclose(np.asarray(res["momenta"]).reshape(-1, 10, 3)[acc], mean[acc],
+                               rtol=1e-6, atol=1e-6 * np.abs(mean).max())
+    st = clo.stats()
+    nc = np.count_nonzero(cand)
+    assert st["candidates"] == nc and st["fallback"] == nc - np.count_nonzero(acc)
+    assert 2 * nc < st["member_rows"] < 3 * nc
+
