@@ -368,6 +368,21 @@ struct RunArgs {
   const uint32_t* sim_list;
 };
 
+// The launch's RunArgs (kernel argument 0, at offset 0 of the kernarg segment)
+// read at the point of use, through an address the compiler cannot see through.
+// The finalize block reads its fields this way: the output pointers and flags
+// only it uses are then not held in scalar registers across the step loop, where
+// they spilled to VGPR lanes and came back with v_readlane every step (CH4:
+// 13.80 -> 13.59 ms).  The refill block keeps the direct reads: its scalar loads
+// at the point of use measured slower (14.02 ms), it runs as often and waits on
+// them before its first force.
+typedef const __attribute__((address_space(4))) RunArgs* ColdArgs;
+__device__ __forceinline__ ColdArgs cold_args() {
+  uint64_t a = reinterpret_cast<uint64_t>(__builtin_amdgcn_kernarg_segment_ptr());
+  asm volatile("" : "+s"(a));
+  return reinterpret_cast<ColdArgs>(a);
+}
+
 struct Totals {
   uint32_t att = 0, acc = 0, refills = 0;
   uint32_t iters = 0;  // loop iterations of the wave (the same on every lane)
@@ -420,6 +435,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         stopped != 0 && (__ballot(active && status == kRunning) == 0 ||
                          __popcll(stopped) + __popcll(__ballot(!active)) >= p.refill_threshold);
     if (fin_now && active && status != kRunning) {
+      const ColdArgs cp = cold_args();
       double ke = 0.0;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -427,7 +443,7 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
         ke += 0.5 * mc.md[i] * (vx * vx + vy * vy + vz * vz);
       }
       const double e0d = static_cast<double>(e0);
-      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0d / ke) : 1.0;
+      const double s = (cp->redistribute && ke > 0.0) ? __builtin_sqrt(e0d / ke) : 1.0;
       const uint64_t base = sg.pos_off + loc * N3;
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -435,28 +451,28 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double pm = ms * static_cast<double>(v[3 * i + c]);
-          p.mom[base + 3 * i + c] = pm;
-          p.mom32[base + 3 * i + c] = static_cast<float>(pm);
+          cp->mom[base + 3 * i + c] = pm;
+          cp->mom32[base + 3 * i + c] = static_cast<float>(pm);
         }
       }
       const uint64_t gs = sg.start + loc;
-      if (p.state_x != nullptr) {
-        T* sx = static_cast<T*>(p.state_x) + base;
-        T* sv = static_cast<T*>(p.state_v) + base;
+      if (cp->state_x != nullptr) {
+        T* sx = static_cast<T*>(cp->state_x) + base;
+        T* sv = static_cast<T*>(cp->state_v) + base;
 #pragma unroll
         for (int q = 0; q < N3; ++q) {
           sx[q] = x[q];
           sv[q] = v[q];
         }
-        p.state_h[gs] = maxT(static_cast<double>(h), p.state_h[gs]);
-        p.state_e0[gs] = e0d;
+        cp->state_h[gs] = maxT(static_cast<double>(h), cp->state_h[gs]);
+        cp->state_e0[gs] = e0d;
       }
-      tot.att += p.resume ? att - p.attempts[gs] : att;
-      p.steps[gs] = steps;
-      p.attempts[gs] = att;
-      p.status[gs] = static_cast<uint8_t>(status);
-      p.t_final[gs] = t;
-      if (p.eerr != nullptr) {
+      tot.att += cp->resume ? att - cp->attempts[gs] : att;
+      cp->steps[gs] = steps;
+      cp->attempts[gs] = att;
+      cp->status[gs] = static_cast<uint8_t>(status);
+      cp->t_final[gs] = t;
+      if (cp->eerr != nullptr) {
         // energy diagnostic (KE + PE - E0) / E0 in FP64, when asked for
         double pe_end = 0.0;
         static_for<0, N>([&](auto I) {
@@ -469,10 +485,10 @@ __device__ __forceinline__ void run_segment(const RunArgs& p, const Segment& sg,
             pe_end = fma(pk.template getd<i, j>(), rsqrt_f64(rx * rx + ry * ry + rz * rz), pe_end);
           });
         });
-        p.eerr[gs] = (e0d != 0.0) ? (ke + pe_end - e0d) / e0d : 0.0;
+        cp->eerr[gs] = (e0d != 0.0) ? (ke + pe_end - e0d) / e0d : 0.0;
       }
-      if (p.trace_h != nullptr)
-        for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
+      if (cp->trace_h != nullptr)
+        for (uint32_t q = steps; q < cp->trace_cap; ++q) cp->trace_h[gs * cp->trace_cap + q] = 0.0;
       active = false;
     }
 

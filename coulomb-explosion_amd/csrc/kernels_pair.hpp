@@ -300,6 +300,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
                          2 * (__popcll(stopped) + __popcll(__ballot(!active) & kLeadMask)) >=
                              p.refill_threshold);
     if (fin_now && active && status != kRunning) {
+      const ColdArgs cp = cold_args();  // the finalize's fields at their point of use
       double ke = 0.0;
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
@@ -309,7 +310,7 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
       }
       ke = pair_sum(ke);
       const double e0d = static_cast<double>(e0);
-      const double s = (p.redistribute && ke > 0.0) ? __builtin_sqrt(e0d / ke) : 1.0;
+      const double s = (cp->redistribute && ke > 0.0) ? __builtin_sqrt(e0d / ke) : 1.0;
       const uint64_t base = sg.pos_off + loc * N3 + hf * NA3;
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
@@ -317,26 +318,26 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const double pm = ms * static_cast<double>(v[3 * i + c]);
-          p.mom[base + 3 * i + c] = pm;
-          p.mom32[base + 3 * i + c] = static_cast<float>(pm);
+          cp->mom[base + 3 * i + c] = pm;
+          cp->mom32[base + 3 * i + c] = static_cast<float>(pm);
         }
       }
       const uint64_t gs = sg.start + loc;
-      if (p.state_x != nullptr) {
-        T* sx = static_cast<T*>(p.state_x) + base;
-        T* sv = static_cast<T*>(p.state_v) + base;
+      if (cp->state_x != nullptr) {
+        T* sx = static_cast<T*>(cp->state_x) + base;
+        T* sv = static_cast<T*>(cp->state_v) + base;
 #pragma unroll
         for (int q = 0; q < NA3; ++q) {
           sx[q] = x[q];
           sv[q] = v[q];
         }
         if (lead) {
-          p.state_h[gs] = maxT(static_cast<double>(h), p.state_h[gs]);
-          p.state_e0[gs] = e0d;
+          cp->state_h[gs] = maxT(static_cast<double>(h), cp->state_h[gs]);
+          cp->state_e0[gs] = e0d;
         }
       }
       double pe_end = 0.0;
-      if (p.eerr != nullptr) {
+      if (cp->eerr != nullptr) {
         // energy diagnostic (KE + PE - E0) / E0 in FP64, both halves
         double xpd[NA3];
 #pragma unroll
@@ -363,14 +364,14 @@ __device__ __forceinline__ void run_segment_pair(const RunArgs& p, const Segment
         pe_end = pair_sum(pe_end);
       }
       if (lead) {
-        tot.att += p.resume ? att - p.attempts[gs] : att;
-        p.steps[gs] = steps;
-        p.attempts[gs] = att;
-        p.status[gs] = static_cast<uint8_t>(status);
-        p.t_final[gs] = t;
-        if (p.eerr != nullptr) p.eerr[gs] = (e0d != 0.0) ? (ke + pe_end - e0d) / e0d : 0.0;
-        if (p.trace_h != nullptr)
-          for (uint32_t q = steps; q < p.trace_cap; ++q) p.trace_h[gs * p.trace_cap + q] = 0.0;
+        tot.att += cp->resume ? att - cp->attempts[gs] : att;
+        cp->steps[gs] = steps;
+        cp->attempts[gs] = att;
+        cp->status[gs] = static_cast<uint8_t>(status);
+        cp->t_final[gs] = t;
+        if (cp->eerr != nullptr) cp->eerr[gs] = (e0d != 0.0) ? (ke + pe_end - e0d) / e0d : 0.0;
+        if (cp->trace_h != nullptr)
+          for (uint32_t q = steps; q < cp->trace_cap; ++q) cp->trace_h[gs * cp->trace_cap + q] = 0.0;
       }
       active = false;
     }
